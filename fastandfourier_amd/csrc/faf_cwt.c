@@ -1,0 +1,580 @@
+/*
+ * faf_cwt.c -- CWT filter bank (C11 host side) and its device launches.
+ *
+ * The bank is built on the host in FP64 at create time, following the
+ * reference's construction (src/faf_cwt.c:104-677: scale geometry, L2
+ * shadow with plateau rescale, residual lowpass, Littlewood-Paley report,
+ * live FP32 table, dual-frame table, wrap check), then uploaded once.  The
+ * forward transform runs as one fused kernel per segment: one rfft, then
+ * per row a one-sided multiply and an inverse FFT.
+ */
+#define _GNU_SOURCE
+#include "faf_internal.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+#define LN2 0.693147180559945309417
+#define MAX_SCALES 512
+
+typedef struct fafi_cwt_bank {
+    faf_cwt_config cfg;      /* resolved config: first member (faf_create_inverse) */
+    size_t n, n_bins, stride, n_scales, n_rows;
+    int has_lp;
+    double *scales, *freqs, *l2, *A, *dual;
+    float *psi32, *dual32;
+    faf_cwt_lp_report report;
+    int inv_kind;
+} fafi_cwt_bank;
+
+/* ------------------------------------------------------------------ */
+/* config defaults (reference src/faf_cwt.c:27-51)                      */
+/* ------------------------------------------------------------------ */
+
+faf_cwt_config faf_cwt_config_init(size_t n) {
+    faf_cwt_config c;
+    memset(&c, 0, sizeof c);
+    c.n = n;
+    c.fs = 1.0;
+    c.precision = FAF_PREC_FP32;
+    c.backend = FAF_BACKEND_AUTO;
+    c.wavelet = FAF_CWT_WAVELET_MORSE;
+    c.norm = FAF_CWT_NORM_L1;
+    c.scale_kind = FAF_CWT_SCALE_GEOMETRIC;
+    c.center_kind = FAF_CWT_CENTER_PEAK;
+    c.voices = 10;
+    c.morlet_mu = 6.0;
+    c.morse_gamma = 3.0;
+    c.morse_beta = 20.0;     /* time-bandwidth beta*gamma = 60 */
+    c.bump_center = 5.0;
+    c.bump_width = 1.0;
+    c.bandpass_peak = 2.0;
+    c.lp_alpha = 0.05;
+    c.lp_beta = 0.05;
+    c.lp_floor = 1e-4;
+    c.flags = FAF_CWT_FLAG_INCLUDE_LOWPASS | FAF_CWT_FLAG_VALIDATE_STRICT;
+    return c;
+}
+
+/* ------------------------------------------------------------------ */
+/* analytic prototypes in the Fourier domain (reference                 */
+/* src/faf_cwt_wavelets.c:13-80); all vanish for omega <= 0             */
+/* ------------------------------------------------------------------ */
+
+static double meyer_nu(double x) {
+    if (x <= 0.0) return 0.0;
+    if (x >= 1.0) return 1.0;
+    return x * x * x * x * (35.0 - 84.0 * x + 70.0 * x * x - 20.0 * x * x * x);
+}
+
+static double proto(double w, const faf_cwt_config *c) {
+    if (w <= 0.0) return 0.0;
+    switch (c->wavelet) {
+    case FAF_CWT_WAVELET_MORLET: {
+        double mu = c->morlet_mu;
+        return exp(-0.5 * (w - mu) * (w - mu)) - exp(-0.5 * mu * mu) * exp(-0.5 * w * w);
+    }
+    case FAF_CWT_WAVELET_MORSE: {
+        /* peak-normalised generalized Morse, evaluated in the log domain */
+        double g = c->morse_gamma, b = c->morse_beta;
+        double wp = pow(b / g, 1.0 / g);
+        double lp = b * log(wp) - pow(wp, g);
+        return exp(b * log(w) - pow(w, g) - lp);
+    }
+    case FAF_CWT_WAVELET_BUMP: {
+        double u = (w - c->bump_center) / c->bump_width;
+        if (fabs(u) >= 1.0) return 0.0;
+        return exp(1.0 - 1.0 / (1.0 - u * u));
+    }
+    case FAF_CWT_WAVELET_SHANNON:
+        return (w >= M_PI / 2.0 && w <= M_PI) ? 1.0 : 0.0;
+    case FAF_CWT_WAVELET_MEYER:
+        if (w < 2.0 * M_PI / 3.0) return 0.0;
+        if (w <= 4.0 * M_PI / 3.0) return sin(0.5 * M_PI * meyer_nu(3.0 * w / (2.0 * M_PI) - 1.0));
+        if (w <= 8.0 * M_PI / 3.0) return cos(0.5 * M_PI * meyer_nu(3.0 * w / (4.0 * M_PI) - 1.0));
+        return 0.0;
+    default:
+        return 0.0;
+    }
+}
+
+static double peak_omega(const faf_cwt_config *c) {
+    switch (c->wavelet) {
+    case FAF_CWT_WAVELET_MORLET: return c->morlet_mu;
+    case FAF_CWT_WAVELET_MORSE: return pow(c->morse_beta / c->morse_gamma, 1.0 / c->morse_gamma);
+    case FAF_CWT_WAVELET_BUMP: return c->bump_center;
+    case FAF_CWT_WAVELET_SHANNON: return 0.75 * M_PI;
+    case FAF_CWT_WAVELET_MEYER: return 4.0 * M_PI / 3.0;
+    default: return 1.0;
+    }
+}
+
+/* energy centroid of |psi|^2 and the admissibility integral: Riemann sums
+ * with step 1e-4 accumulated the same way as reference src/faf_cwt.c:75-98 */
+static double energy_omega(const faf_cwt_config *c) {
+    double dw = 1e-4, num = 0.0, den = 0.0;
+    for (double w = dw; w <= 80.0; w += dw) {
+        double e = proto(w, c);
+        e *= e;
+        num += w * e;
+        den += e;
+    }
+    return den > 0.0 ? num / den : peak_omega(c);
+}
+
+static double admissibility(const faf_cwt_config *c, double xi0) {
+    double dw = 1e-4, sum = 0.0;
+    for (double w = dw; w <= xi0 * 20.0; w += dw) sum += proto(w, c) / w;
+    return sum * dw;
+}
+
+static void bank_free(fafi_cwt_bank *b) {
+    if (!b) return;
+    free(b->scales); free(b->freqs); free(b->l2); free(b->A); free(b->dual);
+    free(b->psi32); free(b->dual32);
+    free(b);
+}
+
+/* ------------------------------------------------------------------ */
+/* bank construction (reference src/faf_cwt.c:104-677)                  */
+/* ------------------------------------------------------------------ */
+
+static fafi_cwt_bank *bank_build(const faf_cwt_config *cfg) {
+    faf_cwt_config c = *cfg;
+    if (c.n < 2 || (c.n & 1) || !fafi_is_5_smooth(c.n)) {
+        fafi_err("cwt: n=%zu not even 5-smooth", c.n);
+        return NULL;
+    }
+    if (c.precision != FAF_PREC_FP32) {
+        fafi_err(c.precision == FAF_PREC_FP64
+                     ? "cwt: this build computes in FP32 only"
+                     : "cwt: precision %d unsupported", (int)c.precision);
+        return NULL;
+    }
+    if (c.voices == 0) { fafi_err("cwt: voices must be >= 1"); return NULL; }
+    if (c.fs <= 0.0) c.fs = 1.0;
+    if (c.wavelet == FAF_CWT_WAVELET_MORSE && (c.morse_gamma <= 0.0 || c.morse_beta <= 0.0)) {
+        fafi_err("cwt: morse gamma/beta must be > 0");
+        return NULL;
+    }
+    if (c.wavelet == FAF_CWT_WAVELET_MORLET && c.morlet_mu <= 0.0) {
+        fafi_err("cwt: morlet mu must be > 0");
+        return NULL;
+    }
+    if (c.wavelet == FAF_CWT_WAVELET_BUMP && c.bump_width <= 0.0) {
+        fafi_err("cwt: bump width must be > 0");
+        return NULL;
+    }
+    if (c.flags & FAF_CWT_FLAG_FIXED_LOWPASS) {
+        fafi_err("cwt: FAF_CWT_FLAG_FIXED_LOWPASS is reserved");
+        return NULL;
+    }
+    double xi0 = c.center_kind == FAF_CWT_CENTER_ENERGY ? energy_omega(&c) : peak_omega(&c);
+    if (!(xi0 > 0.0) || xi0 != xi0) {
+        fafi_err("cwt: prototype center frequency is not finite");
+        return NULL;
+    }
+    const size_t n = c.n, nb = n / 2 + 1, stride = (nb + 15) & ~(size_t)15;
+    const double nyq = c.fs / 2.0;
+    if (c.fmax <= 0.0) c.fmax = 0.45 * nyq;
+    if (c.fmin <= 0.0) c.fmin = 16.0 * c.fs / (double)n;
+    if (c.fmin >= c.fmax) { fafi_err("cwt: fmin >= fmax"); return NULL; }
+    if (c.fmax >= nyq) { fafi_err("cwt: fmax at or above Nyquist"); return NULL; }
+
+    size_t J;
+    if (c.scale_kind == FAF_CWT_SCALE_GEOMETRIC) {
+        double oct = log(c.fmax / c.fmin) / LN2;
+        if (!(oct >= 0.0)) { fafi_err("cwt: fmin/fmax do not form a valid octave span"); return NULL; }
+        J = (size_t)floor((double)c.voices * oct + 1.0 + 1e-12);
+        if (J < 1) J = 1;
+    } else {
+        J = c.voices < 2 ? 2 : (size_t)c.voices;
+    }
+    if (J > MAX_SCALES) { fafi_err("cwt: too many scales (%zu > %d)", J, MAX_SCALES); return NULL; }
+
+    fafi_cwt_bank *b = calloc(1, sizeof *b);
+    const int lp = (c.flags & FAF_CWT_FLAG_INCLUDE_LOWPASS) ? 1 : 0;
+    const size_t rows = J + (size_t)lp, off = (size_t)lp;
+    b->cfg = c;
+    b->n = n; b->n_bins = nb; b->stride = stride; b->n_scales = J; b->n_rows = rows;
+    b->has_lp = lp;
+    b->scales = calloc(J, sizeof(double));
+    b->freqs = calloc(J, sizeof(double));
+    b->l2 = calloc(rows * stride, sizeof(double));
+    b->A = calloc(nb, sizeof(double));
+    b->dual = calloc(rows * stride, sizeof(double));
+    b->psi32 = calloc(rows * stride, sizeof(float));
+    b->dual32 = calloc(rows * stride, sizeof(float));
+    double *xi = calloc(nb, sizeof(double));
+    for (size_t k = 0; k < nb; ++k) xi[k] = 2.0 * M_PI * (double)k / (double)n;
+
+    for (size_t j = 0; j < J; ++j) {
+        double f = c.scale_kind == FAF_CWT_SCALE_GEOMETRIC
+                       ? c.fmin * pow(2.0, (double)j / (double)c.voices)
+                       : c.fmin + (double)j * (c.fmax - c.fmin) / (double)(J > 1 ? J - 1 : 1);
+        b->freqs[j] = f;
+        b->scales[j] = xi0 / (2.0 * M_PI * f / c.fs);
+    }
+    /* L2 shadow: rows coarse -> fine after the lowpass slot */
+    for (size_t j = 0; j < J; ++j) {
+        double *row = b->l2 + (off + j) * stride;
+        for (size_t k = 0; k < nb; ++k) row[k] = proto(b->scales[j] * xi[k], &c);
+        row[0] = 0.0;
+        if (nb > 1) row[nb - 1] = 0.0;
+    }
+    if (c.scale_kind == FAF_CWT_SCALE_GEOMETRIC) {
+        /* rescale so the plateau of sum_j psi^2 (bins within 5% of its peak) is 1 */
+        double amax = 0.0, mean = 0.0;
+        double *ap = calloc(nb, sizeof(double));
+        size_t cnt = 0;
+        for (size_t k = 0; k < nb; ++k) {
+            double s = 0.0;
+            for (size_t j = 0; j < J; ++j) {
+                double v = b->l2[(off + j) * stride + k];
+                s += v * v;
+            }
+            ap[k] = s;
+            if (s > amax) amax = s;
+        }
+        for (size_t k = 0; k < nb; ++k)
+            if (ap[k] >= 0.95 * amax) { mean += ap[k]; ++cnt; }
+        if (cnt) mean /= (double)cnt;
+        if (mean <= 0.0) mean = 1.0;
+        double r = 1.0 / sqrt(mean);
+        for (size_t j = 0; j < J; ++j)
+            for (size_t k = 0; k < nb; ++k) b->l2[(off + j) * stride + k] *= r;
+        free(ap);
+    }
+    if (lp) {
+        for (size_t k = 0; k < nb; ++k) {
+            double s = 0.0;
+            for (size_t j = 0; j < J; ++j) {
+                double v = b->l2[(off + j) * stride + k];
+                s += v * v;
+            }
+            b->l2[k] = 1.0 - s > 0.0 ? sqrt(1.0 - s) : 0.0;
+        }
+    }
+    for (size_t k = 0; k < nb; ++k) {
+        double s = 0.0;
+        for (size_t r = 0; r < rows; ++r) s += b->l2[r * stride + k] * b->l2[r * stride + k];
+        b->A[k] = s;
+    }
+    /* live FP32 analysis table */
+    for (size_t r = 0; r < rows; ++r) {
+        float *live = b->psi32 + r * stride;
+        const double *l2 = b->l2 + r * stride;
+        if ((int)r < lp) {
+            for (size_t k = 0; k < nb; ++k) live[k] = (float)l2[k];
+            continue;
+        }
+        double s = b->scales[r - off];
+        if (c.norm == FAF_CWT_NORM_L2) {
+            for (size_t k = 0; k < nb; ++k) live[k] = (float)l2[k];
+        } else {
+            float peak = 0.0f;
+            for (size_t k = 0; k < nb; ++k) {
+                live[k] = (float)proto(s * xi[k], &c);
+                if (live[k] > peak) peak = live[k];
+            }
+            live[0] = 0.0f;
+            if (c.norm == FAF_CWT_NORM_BANDPASS && peak > 0.0f) {
+                float sc = (float)(c.bandpass_peak / (double)peak);
+                for (size_t k = 0; k < nb; ++k) live[k] *= sc;
+            }
+        }
+        if (nb > 1) live[nb - 1] = 0.0f;
+    }
+    /* dual frame from the live table: dual = psi / (sum_r psi^2 + eps) */
+    for (size_t k = 0; k < nb; ++k) {
+        double s = 0.0;
+        for (size_t r = 0; r < rows; ++r) {
+            double v = b->psi32[r * stride + k];
+            s += v * v;
+        }
+        for (size_t r = 0; r < rows; ++r) {
+            double v = b->psi32[r * stride + k];
+            double d = s > 0.0 ? v / (s + 1e-6) : 0.0;
+            b->dual[r * stride + k] = d;
+            b->dual32[r * stride + k] = (float)d;
+        }
+    }
+    /* Littlewood-Paley report */
+    faf_cwt_lp_report *rep = &b->report;
+    rep->n = n; rep->n_bins = nb; rep->n_scales = J; rep->has_lowpass = lp;
+    rep->alpha = c.lp_alpha; rep->beta = c.lp_beta; rep->A = b->A;
+    size_t klo = 1;
+    if (!lp) {
+        const double *lo = b->l2 + off * stride;
+        double pk = 0.0;
+        for (size_t k = 0; k < nb; ++k) if (lo[k] > pk) pk = lo[k];
+        for (klo = 1; klo < nb; ++klo) if (lo[klo] >= pk * c.lp_floor) break;
+    }
+    size_t khi;
+    {
+        const double *hi = b->l2 + (off + J - 1) * stride;
+        double pk = 0.0;
+        for (size_t k = 0; k < nb; ++k) if (hi[k] > pk) pk = hi[k];
+        for (khi = nb - 2; khi > klo; --khi) if (hi[khi] >= pk * c.lp_floor) break;
+    }
+    rep->k_lo = klo; rep->k_hi = khi;
+    rep->hole_bins_dc = klo;
+    rep->hole_bins_nyq = (nb - 1) - khi;
+    double mx = 0.0, sum = 0.0, amin = 1e30, amx = -1e30;
+    size_t cnt = 0;
+    for (size_t k = klo; k <= khi; ++k) {
+        double a = b->A[k], d = fabs(a - 1.0);
+        if (d > mx) mx = d;
+        sum += d;
+        if (a < amin) amin = a;
+        if (a > amx) amx = a;
+        ++cnt;
+    }
+    if (!cnt) { amin = 0.0; amx = 0.0; }
+    rep->max_abs_dev = mx;
+    rep->mean_abs_dev = cnt ? sum / (double)cnt : 0.0;
+    rep->min_A = amin; rep->max_A = amx;
+    rep->frame_cond = amin > 0.0 ? amx / amin : INFINITY;
+    double mdc = 0.0;
+    for (size_t j = 0; j < J; ++j) {
+        double v = fabs(b->l2[(off + j) * stride]);
+        if (v > mdc) mdc = v;
+    }
+    rep->max_dc_wavelet = mdc;
+    rep->admissibility_C = admissibility(&c, xi0);
+    rep->passed = (mx <= c.lp_alpha && rep->mean_abs_dev <= c.lp_beta && cnt > 0) ? 1 : 0;
+    free(xi);
+    const int strict = (c.flags & FAF_CWT_FLAG_VALIDATE_STRICT) &&
+                       !(c.flags & FAF_CWT_FLAG_ALLOW_UNTILED);
+    if (strict && !rep->passed) {
+        fafi_err("cwt: LP bound violated max|A-1|=%g on [%zu,%zu]", mx, klo, khi);
+        bank_free(b);
+        return NULL;
+    }
+    /* wrap check: energy of the coarsest L2 kernel near t = n/2, from an
+     * FP64 inverse DFT of its (real, even) spectrum */
+    {
+        double *re = calloc(n, sizeof(double)), *im = calloc(n, sizeof(double));
+        const double *psi = b->l2 + off * stride;
+        for (size_t k = 0; k < nb; ++k) {
+            re[k] = psi[k];
+            if (k && k < n - k) re[n - k] = psi[k];
+        }
+        fafi_fft64(n, 1, re, im);
+        double tot = 0.0, wrap = 0.0;
+        size_t half = n / 2, win = n / 32 ? n / 32 : 1;
+        for (size_t i = 0; i < n; ++i) tot += re[i] * re[i];
+        for (size_t i = half - win; i < half + win && i < n; ++i) wrap += re[i] * re[i];
+        rep->max_wrap_energy = tot > 0.0 ? wrap / tot : 0.0;
+        free(re); free(im);
+        if (strict && rep->max_wrap_energy > 1e-3) {
+            fafi_err("cwt: coarsest scale wraps (energy=%g)", rep->max_wrap_energy);
+            bank_free(b);
+            return NULL;
+        }
+    }
+    return b;
+}
+
+/* ------------------------------------------------------------------ */
+/* transforms                                                          */
+/* ------------------------------------------------------------------ */
+
+static void *upload_f32(const float *h, size_t count) {
+    void *d = fafk_malloc(count * sizeof(float));
+    if (!d || fafk_h2d(d, h, count * sizeof(float), NULL) || fafk_sync(NULL)) {
+        fafi_err("%s", fafk_last_error());
+        fafk_free(d);
+        return NULL;
+    }
+    return d;
+}
+
+int fafi_cwt_build(fafh_plan *p, const faf_cwt_config *cfg, int inverse_kind) {
+    fafi_cwt_bank *b = bank_build(cfg);
+    if (!b) return -1;
+    p->bank = b;
+    b->inv_kind = inverse_kind;
+    size_t n = b->n, nc = n / 2;
+    p->n = n;
+    p->psi_stride = (long)b->stride;
+    p->rows_total = (int)b->n_rows;
+    p->post = fafi_kiss_table(nc, 0);
+    p->pre = fafi_kiss_table(nc, 1);
+    if (!p->post || !p->pre) return -1;
+    if (!fafi_radix_fits(nc, 0) || fafi_make_fft(nc, 0, 0, NULL, 0, &p->a)) {
+        fafi_err("cwt: length %zu does not fit one workgroup", n);
+        return -1;
+    }
+    if (inverse_kind < 0) {
+        /* forward: analytic rows need a length-n inverse FFT */
+        if (!fafi_radix_fits(n, 0) || fafi_make_fft(n, 1, 0, NULL, 0, &p->full_inv)) {
+            fafi_err("cwt: length %zu does not fit one workgroup", n);
+            return -1;
+        }
+        p->psi_dev = upload_f32(b->psi32, b->n_rows * b->stride);
+        p->kind = PK_CWT;
+        p->scale = (float)(1.0 / (double)n);
+        return p->psi_dev ? 0 : -1;
+    }
+    if (fafi_make_fft(nc, 1, 0, NULL, 0, &p->b)) return -1;
+    p->dual_dev = upload_f32(b->dual32, b->n_rows * b->stride);
+    p->kind = PK_ICWT;
+    p->scale = (float)(1.0 / (double)n);
+    return p->dual_dev ? 0 : -1;
+}
+
+void fafi_cwt_free(fafh_plan *p) {
+    if (!p) return;
+    bank_free(p->bank);
+    p->bank = NULL;
+    fafk_free(p->psi_dev);
+    fafk_free(p->dual_dev);
+    p->psi_dev = p->dual_dev = NULL;
+}
+
+int fafi_run_cwt(fafh_plan *p, const float *x, long idist, float *out, long odist,
+                 long count, int complex_out, void *stream) {
+    int rows = p->rows_total;
+    /* enough workgroups to fill the device: split the rows of a segment
+     * across workgroups when there are few segments */
+    long want = 2 * (long)fafi_cus();
+    int groups = (int)((want + count - 1) / count);
+    if (groups < 1) groups = 1;
+    if (groups > rows) groups = rows;
+    int rpw = (rows + groups - 1) / groups;
+    int rc = fafk_cwt(&p->a, &p->full_inv, !complex_out, p->post, p->psi_dev, p->psi_stride,
+                      rows, rpw, x, idist, out, complex_out ? odist * 1 : odist, count,
+                      p->scale, stream);
+    if (rc) fafi_err("%s", fafk_last_error());
+    return rc;
+}
+
+int fafi_run_icwt(fafh_plan *p, const float *w, long idist, float *y, long odist, long count,
+                  void *stream) {
+    fafi_cwt_bank *b = p->bank;
+    if (b->inv_kind == FAF_CWT_INV_L1) {
+        fafi_err("cwt: the L1 one-integral inverse is not on the device path; use the "
+                 "dual-frame inverse");
+        return -1;
+    }
+    int rc = fafk_icwt(&p->a, &p->b, p->post, p->pre, p->dual_dev, p->psi_stride,
+                       p->rows_total, w, idist, y, odist, count, p->scale, stream);
+    if (rc) fafi_err("%s", fafk_last_error());
+    return rc;
+}
+
+static faf_transform *cwt_transform(const faf_cwt_config *cfg, int inverse_kind) {
+    if (!cfg) { fafi_err("cwt: null argument"); return NULL; }
+    if (fafi_runtime_ready()) return NULL;
+    faf_transform *t = calloc(1, sizeof(faf_transform));
+    fafh_plan *p = fafi_plan_new();
+    if (!t || !p) { free(t); free(p); fafi_err("cwt: alloc failed"); return NULL; }
+    t->amd = p;
+    if (fafi_cwt_build(p, cfg, inverse_kind)) {
+        fafi_plan_free(p);
+        free(t);
+        return NULL;
+    }
+    fafi_cwt_bank *b = p->bank;
+    t->type = inverse_kind < 0 ? FAF_TRANSFORM_CWT : FAF_TRANSFORM_ICWT;
+    t->n = b->n;
+    t->precision = b->cfg.precision;
+    t->cfg = faf_config_init(b->n);
+    t->cfg.precision = b->cfg.precision;
+    t->cfg.layout = FAF_LAYOUT_REAL;
+    t->cfg.norm = FAF_NORM_NONE;
+    t->cfg.backend = cfg->backend;
+    if (inverse_kind >= 0) t->cfg.dir = FAF_DIR_INVERSE;
+    t->user_aux_n = b->n_rows;
+    return t;
+}
+
+faf_transform *faf_create_cwt(const faf_cwt_config *cfg) { return cwt_transform(cfg, -1); }
+
+faf_transform *faf_create_icwt(const faf_cwt_config *cfg, faf_cwt_inverse_kind kind) {
+    if (!cfg) { fafi_err("cwt: null argument"); return NULL; }
+    if (kind == FAF_CWT_INV_L1 && cfg->norm != FAF_CWT_NORM_L1) {
+        fafi_err("cwt: L1 inverse requires L1 bank");
+        return NULL;
+    }
+    return cwt_transform(cfg, (int)kind);
+}
+
+/* ------------------------------------------------------------------ */
+/* queries (reference src/faf_cwt.c:995-1079)                           */
+/* ------------------------------------------------------------------ */
+
+static const fafi_cwt_bank *bank_of(const faf_transform *t) {
+    if (!t || (t->type != FAF_TRANSFORM_CWT && t->type != FAF_TRANSFORM_ICWT) || !t->amd)
+        return NULL;
+    return ((const fafh_plan *)t->amd)->bank;
+}
+
+const faf_cwt_lp_report *faf_cwt_bank_report(const faf_transform *t) {
+    const fafi_cwt_bank *b = bank_of(t);
+    return b ? &b->report : NULL;
+}
+size_t faf_cwt_n_scales(const faf_transform *t) { const fafi_cwt_bank *b = bank_of(t); return b ? b->n_scales : 0; }
+size_t faf_cwt_n_rows(const faf_transform *t) { const fafi_cwt_bank *b = bank_of(t); return b ? b->n_rows : 0; }
+size_t faf_cwt_n_bins(const faf_transform *t) { const fafi_cwt_bank *b = bank_of(t); return b ? b->n_bins : 0; }
+int faf_cwt_has_lowpass(const faf_transform *t) { const fafi_cwt_bank *b = bank_of(t); return b ? b->has_lp : 0; }
+
+int faf_cwt_freqs(const faf_transform *t, double *hz, size_t cap) {
+    const fafi_cwt_bank *b = bank_of(t);
+    if (!b || !hz) return -1;
+    size_t m = b->n_scales < cap ? b->n_scales : cap;
+    memcpy(hz, b->freqs, m * sizeof(double));
+    return (int)m;
+}
+
+int faf_cwt_scales(const faf_transform *t, double *s, size_t cap) {
+    const fafi_cwt_bank *b = bank_of(t);
+    if (!b || !s) return -1;
+    size_t m = b->n_scales < cap ? b->n_scales : cap;
+    memcpy(s, b->scales, m * sizeof(double));
+    return (int)m;
+}
+
+void faf_cwt_report_fprint(FILE *fp, const faf_cwt_lp_report *r) {
+    if (!fp || !r) return;
+    fprintf(fp, "CWT LP Report:\n");
+    fprintf(fp, "  n=%zu  n_bins=%zu  n_scales=%zu  lowpass=%d\n", r->n, r->n_bins,
+            r->n_scales, r->has_lowpass);
+    fprintf(fp, "  passed=%d  max|A-1|=%.6g  mean|A-1|=%.6g\n", r->passed, r->max_abs_dev,
+            r->mean_abs_dev);
+    fprintf(fp, "  A range: [%.6g, %.6g]  cond=%.4g\n", r->min_A, r->max_A, r->frame_cond);
+    fprintf(fp, "  certified band: [%zu, %zu]\n", r->k_lo, r->k_hi);
+    fprintf(fp, "  holes: DC=%zu bins, Nyquist=%zu bins\n", r->hole_bins_dc, r->hole_bins_nyq);
+    fprintf(fp, "  max DC wavelet=%.3e  wrap energy=%.3e\n", r->max_dc_wavelet,
+            r->max_wrap_energy);
+    fprintf(fp, "  admissibility C=%.6g\n", r->admissibility_C);
+}
+
+const float *faf_cwt_psi_f32(const faf_transform *t, size_t idx) {
+    const fafi_cwt_bank *b = bank_of(t);
+    if (!b) return NULL;
+    size_t row = idx + (size_t)b->has_lp;
+    return row < b->n_rows ? b->psi32 + row * b->stride : NULL;
+}
+
+const double *faf_cwt_psi_f64(const faf_transform *t, size_t idx) {
+    (void)t; (void)idx;
+    return NULL;   /* FP64 banks are not built by this FP32 device library */
+}
+
+const float *faf_cwt_phi_f32(const faf_transform *t) {
+    const fafi_cwt_bank *b = bank_of(t);
+    return (b && b->has_lp) ? b->psi32 : NULL;
+}
+
+const double *faf_cwt_phi_f64(const faf_transform *t) { (void)t; return NULL; }
+
+/* dual table accessor for tests and callers that build their own inverse */
+const float *faf_amd_cwt_dual_f32(const faf_transform *t, size_t row) {
+    const fafi_cwt_bank *b = bank_of(t);
+    return (b && row < b->n_rows) ? b->dual32 + row * b->stride : NULL;
+}

@@ -1,0 +1,88 @@
+/*
+ * faf_dev.h -- device-plan records shared by the C11 host code and the HIP
+ * kernels.  Plain C, fixed-width fields, passed to kernels by value.
+ *
+ * A Stockham plan describes one complex FFT of length n computed as a
+ * sequence of radix-r passes over LDS.  For pass s with radix r and span
+ * ns (product of the radices before it), butterfly j in [0, n/r) reads
+ * x[j + t*n/r] (t < r), multiplies leg t by W_n^(t*k*tw_stride) with
+ * k = j mod ns, runs an r-point DFT and writes y[(j-k)*r + k + t*ns].
+ * The first pass reads in natural order and the last pass writes in
+ * natural order (self-sorting), so no digit-reversal permutation exists.
+ */
+#ifndef FAF_DEV_H
+#define FAF_DEV_H
+#include <stdint.h>
+
+#define FAFD_MAX_STAGES 16
+#define FAFD_MAXE 16                 /* complex values per thread per pass */
+#define FAFD_MAX_LDS_BYTES 163840    /* gfx950 LDS per workgroup */
+#define FAFD_MAX_OPS 8               /* spectral ops in a fused pipeline */
+
+typedef struct fafd_stage {
+    int32_t radix;
+    int32_t ns;          /* span before this pass */
+    uint32_t ns_magic;   /* ceil(2^32/ns): j/ns == umulhi(j, ns_magic) for j,ns <= 2^16 */
+    int32_t tw_stride;   /* n / (ns*radix) */
+} fafd_stage;
+
+typedef struct fafd_fft {
+    int32_t n;           /* complex length */
+    int32_t nstages;
+    int32_t threads;     /* threads cooperating on one signal */
+    int32_t plane;       /* LDS floats per plane (n plus bank padding) */
+    fafd_stage st[FAFD_MAX_STAGES];
+    const void *tw;      /* device float2[n]: W_n^m for this direction */
+} fafd_fft;
+
+/* spectral op codes of the fused pipeline (reference include/chirp.h:36-38) */
+enum { FAFD_OP_NONE = 0, FAFD_OP_CONJ = 1, FAFD_OP_MUL = 2, FAFD_OP_BANDPASS = 3 };
+
+typedef struct fafd_ops {
+    int32_t nops;
+    int32_t code[FAFD_MAX_OPS];
+    uint32_t a1[FAFD_MAX_OPS];
+    uint32_t a2[FAFD_MAX_OPS];
+    const float *h_re;   /* bound spectrum for MUL (device) */
+    const float *h_im;
+    int64_t h_bins;
+} fafd_ops;
+
+/* plan kinds of the persistent ragged launch */
+enum { FAFD_KIND_STOCKHAM = 0, FAFD_KIND_RADER = 1, FAFD_KIND_BLUESTEIN = 2 };
+
+typedef struct fafd_rader {
+    int32_t p;           /* prime length */
+    const int32_t *rin;  /* device: g^j mod p, j < p-1 */
+    const int32_t *rout; /* device: g^-j mod p */
+    const void *b;       /* device float2[p-1]: FFT_{p-1}(w^rout) / (p-1) */
+} fafd_rader;
+
+typedef struct fafd_blue {
+    int32_t n;           /* signal length */
+    int32_t m;           /* padded 5-smooth length >= 2n-1 */
+    const void *chirp;   /* device float2[n]: exp(-+ i pi k^2/n), k^2 reduced mod 2n */
+    const void *b;       /* device float2[m]: FFT_m(conj chirp pattern) / m */
+} fafd_blue;
+
+/* one entry of the ragged plan table (device resident) */
+typedef struct fafd_rplan {
+    int32_t kind;
+    int32_t len;         /* signal length */
+    int32_t threads;     /* threads per signal */
+    int32_t spb;         /* signals per workgroup */
+    fafd_fft fwd;        /* main or inner forward FFT */
+    fafd_fft inv;        /* inner inverse FFT (Rader / Bluestein) */
+    fafd_rader rader;
+    fafd_blue blue;
+} fafd_rplan;
+
+/* one unit of ragged work: `count` signals of one plan */
+typedef struct fafd_unit {
+    int32_t plan;
+    int32_t count;
+    int32_t first;       /* index into the sorted signal list */
+    int32_t pad;
+} fafd_unit;
+
+#endif

@@ -1,0 +1,117 @@
+/*
+ * faf_internal.h -- host-side internals of the MI355X transform library
+ * (C11).  The public ABI is include/fastandfourier.h + include/faf_amd.h.
+ */
+#ifndef FAF_INTERNAL_H
+#define FAF_INTERNAL_H
+#include "fastandfourier.h"
+#include "faf_amd.h"
+#include "faf_dev.h"
+#include "faf_kern.h"
+
+/* ---- errors ---- */
+void fafi_err(const char *fmt, ...);
+
+/* ---- integer size policy (reference src/faf_fft_stage.c:25-135) ---- */
+int fafi_is_5_smooth(size_t n);
+int fafi_is_7_smooth(size_t n);
+int fafi_is_prime(size_t n);
+int fafi_rader_eligible(size_t n);
+size_t fafi_primitive_root(size_t p);
+size_t fafi_next_5_smooth(size_t n);
+int fafi_is_codelet_size(size_t n);
+
+/* ---- device runtime ---- */
+int fafi_runtime_ready(void);         /* 0 ok, -1 with error set */
+void *fafi_stream(void);              /* current thread's stream (NULL = default) */
+int fafi_sync_mode(void);
+size_t fafi_cus(void);
+
+/* ---- plan building (faf_plan.c) ---- */
+int fafi_radix_fits(size_t n, int allow7);             /* one-workgroup LDS plan exists */
+int fafi_make_fft(size_t n, int inv, int allow7, const int *order, int norder,
+                  fafd_fft *out);                       /* 0 ok */
+const void *fafi_twiddles(size_t n, int inv);           /* device float2[n], cached */
+const void *fafi_kiss_table(size_t nc, int inv);        /* device float2[nc/2+1], cached */
+int fafi_default_spb(const fafd_fft *P, size_t extra_per_signal);
+int fafi_fft64(size_t n, int inv, double *re, double *im);   /* host FP64 FFT, in place */
+int fafi_split_4step(size_t n, int *n1, int *n2);
+
+/* autotuned launch shape for (kind, n); 0 when unknown */
+int fafi_tuned_spb(int kind, size_t n);
+void fafi_tune_record(int kind, size_t n, int spb, double gbps);
+
+/* ---- the device plan behind faf_transform.amd ---- */
+typedef enum {
+    PK_NONE = 0, PK_C2C, PK_C2C_4S, PK_RADER, PK_BLUE, PK_R2C, PK_C2R, PK_R2C_4S, PK_C2R_4S,
+    PK_PIPE, PK_STFT, PK_CWT, PK_ICWT
+} fafh_kind;
+
+typedef struct fafh_plan {
+    fafh_kind kind;
+    int inverse;
+    size_t n;                 /* logical length */
+    fafd_fft a, b;            /* main / inner forward, inner inverse, or 4-step n1,n2 */
+    int n1, n2, cols, rows;
+    const void *twn;          /* four-step W_n table */
+    const void *post, *pre;   /* Kiss tables */
+    fafd_rader rader;
+    fafd_blue blue;
+    void *rader_mem[3];
+    void *blue_mem[2];
+    int spb;
+    float scale;
+    int interleaved;          /* buffers are interleaved complex */
+    void *ws;                 /* device workspace */
+    size_t ws_bytes;
+    void *stg;                /* device staging for host buffers */
+    size_t stg_bytes;
+    /* pipeline */
+    fafd_ops ops;
+    void *h_dev;
+    size_t h_dev_cap;
+    const void *h_host_re, *h_host_im;
+    size_t h_host_bins;
+    /* framed STFT */
+    void *win_dev;
+    size_t win, hop;
+    /* CWT */
+    void *bank;               /* fafi_cwt_bank* */
+    void *psi_dev;            /* device float[rows][stride] (forward) */
+    void *dual_dev;           /* device float[rows][stride] (dual inverse) */
+    long psi_stride;
+    int rows_total, rows_per_wg;
+    fafd_fft full_inv;        /* length-n inverse for analytic CWT rows */
+} fafh_plan;
+
+fafh_plan *fafi_plan_new(void);
+void fafi_plan_free(fafh_plan *p);
+void *fafi_workspace(fafh_plan *p, size_t bytes);
+void *fafi_staging(fafh_plan *p, size_t bytes);
+
+/* build device plans for the transform kinds (faf_plan.c) */
+int fafi_plan_c2c(fafh_plan *p, size_t n, int inv, int bluestein);
+int fafi_plan_real(fafh_plan *p, size_t n, int inv);
+int fafi_plan_pipe(fafh_plan *p, size_t n);
+int fafi_plan_stft(fafh_plan *p, size_t win, size_t hop);
+
+/* launches on device buffers (faf_launch.c); dists in elements */
+int fafi_run_c2c(fafh_plan *p, const float *ir, const float *ii, long idist, float *orr,
+                 float *oi, long odist, long count, void *stream);
+int fafi_run_r2c(fafh_plan *p, const float *x, long idist, float *orr, float *oi, long odist,
+                 long count, void *stream);
+int fafi_run_c2r(fafh_plan *p, const float *ir, const float *ii, long idist, float *y,
+                 long odist, long count, void *stream);
+int fafi_run_pipe(fafh_plan *p, const float *x, long idist, float *y, long odist, long count,
+                  void *stream);
+int fafi_pipe_refresh(fafh_plan *p, void *stream);
+
+/* CWT bank (faf_cwt.c) */
+int fafi_cwt_build(fafh_plan *p, const faf_cwt_config *cfg, int inverse_kind);
+void fafi_cwt_free(fafh_plan *p);
+int fafi_run_cwt(fafh_plan *p, const float *x, long idist, float *out, long odist,
+                 long count, int complex_out, void *stream);
+int fafi_run_icwt(fafh_plan *p, const float *w, long idist, float *y, long odist,
+                  long count, void *stream);
+
+#endif

@@ -1,0 +1,89 @@
+/*
+ * faf_kern.h -- C declarations of the kernel launch layer (faf_kernels.hip).
+ * Included by the C11 host code; all arguments are plain pointers, sizes
+ * and the plan records of faf_dev.h.
+ */
+#ifndef FAF_KERN_H
+#define FAF_KERN_H
+#include <stddef.h>
+#include <stdint.h>
+#include "faf_dev.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *fafk_last_error(void);
+int fafk_init(void);
+void *fafk_malloc(size_t bytes);
+void fafk_free(void *p);
+int fafk_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int fafk_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int fafk_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int fafk_sync(void *stream);
+int fafk_is_device_ptr(const void *p);
+void *fafk_stream_create(void);
+void fafk_stream_destroy(void *s);
+void *fafk_event_create(void);
+void fafk_event_destroy(void *ev);
+int fafk_event_record(void *ev, void *stream);
+int fafk_event_query(void *ev);
+int fafk_event_sync(void *ev);
+float fafk_event_ms(void *a, void *b);
+int fafk_device_info(int *cus, size_t *lds_per_block, size_t *total_mem);
+int fafk_fill_u32(unsigned *p, int n, unsigned v, void *stream);
+int fafk_window_inter(const float *x, long in_dist, const float *w, long n, float *y,
+                      long count, void *stream);
+
+int fafk_c2c(const fafd_fft *P, int inv, int inter, int spb, const float *in_re,
+             const float *in_im, long in_dist, float *out_re, float *out_im, long out_dist,
+             long count, float scale, void *stream);
+int fafk_r2c(const fafd_fft *P, int inter_out, int spb, const void *post, const float *x,
+             long in_dist, float *out_re, float *out_im, long out_dist, long count,
+             float scale, void *stream);
+int fafk_c2r(const fafd_fft *P, int inter_in, int spb, const void *pre, const float *in_re,
+             const float *in_im, long in_dist, float *y, long out_dist, long count,
+             float scale, void *stream);
+int fafk_pipe(const fafd_fft *Pf, const fafd_fft *Pi, int spb, const void *post,
+              const void *pre, const fafd_ops *ops, const float *x, long in_dist, float *y,
+              long out_dist, long count, float scale, void *stream);
+int fafk_stft(const fafd_fft *P, int spb, const void *post, const float *win, const float *x,
+              long rec_len, long rec_dist, long hop, long frames, float *out_re,
+              float *out_im, long frame_dist, long rec_out_dist, long count, void *stream);
+int fafk_cwt(const fafd_fft *Ph, const fafd_fft *Pn, int real_out, const void *post,
+             const float *psi, long psi_stride, int rows, int rows_per_wg, const float *x,
+             long in_dist, float *out, long out_dist, long count, float scale, void *stream);
+int fafk_icwt(const fafd_fft *Pf, const fafd_fft *Pi, const void *post, const void *pre,
+              const float *dual, long dual_stride, int rows, const float *w, long in_dist,
+              float *y, long out_dist, long count, float scale, void *stream);
+int fafk_fs_col(const fafd_fft *P1, int inv, int in_kind, int n2, int cols, const void *twn,
+                const float *in_re, const float *in_im, long in_dist, float *t_re,
+                float *t_im, long t_dist, long count, void *stream);
+int fafk_fs_row(const fafd_fft *P2, int inv, int out_kind, int n1, int rows,
+                const float *t_re, const float *t_im, long t_dist, float *out_re,
+                float *out_im, long out_dist, long count, float scale, void *stream);
+int fafk_r2c_post(int nc, const void *post, int inter_out, const float *z_re,
+                  const float *z_im, long z_dist, float *out_re, float *out_im,
+                  long out_dist, long count, float scale, void *stream);
+int fafk_c2r_pre(int nc, const void *pre, int inter_in, const float *in_re,
+                 const float *in_im, long in_dist, float *f_re, float *f_im, long f_dist,
+                 long count, void *stream);
+int fafk_rader(const fafd_fft *Pf, const fafd_fft *Pi, const fafd_rader *R, int spb,
+               const float *in_re, const float *in_im, long in_dist, float *out_re,
+               float *out_im, long out_dist, long count, float scale, void *stream);
+int fafk_blue(const fafd_fft *Pf, const fafd_fft *Pi, const fafd_blue *Bq, int spb,
+              const float *in_re, const float *in_im, long in_dist, float *out_re,
+              float *out_im, long out_dist, long count, float scale, void *stream);
+int fafk_ragged(const void *plans, const void *units, int nunits, const int32_t *order,
+                const long *offsets, const float *in_re, const float *in_im, float *out_re,
+                float *out_im, unsigned *counter, int grid, int block, size_t lds,
+                void *stream);
+
+/* four-step I/O kinds (match the engine's enums) */
+enum { FAFK_FS_IN_SPLIT = 0, FAFK_FS_IN_PAIR = 1, FAFK_FS_IN_INTER = 2 };
+enum { FAFK_FS_OUT_SPLIT = 0, FAFK_FS_OUT_PAIR = 1, FAFK_FS_OUT_INTER = 2 };
+
+#ifdef __cplusplus
+}
+#endif
+#endif

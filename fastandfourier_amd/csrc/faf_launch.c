@@ -1,0 +1,146 @@
+/*
+ * faf_launch.c -- run a planned transform over device buffers (C11).
+ *
+ * Lengths that fit one workgroup's LDS run as a single kernel launch per
+ * batch.  Longer lengths use the four-step split n = n1 * n2 (pass A over
+ * columns, pass B over rows) through a device workspace, processed in
+ * chunks of signals so the intermediate stays small enough to be served
+ * largely from the MI355X's 256 MB last-level cache.
+ */
+#include "faf_internal.h"
+#include <string.h>
+
+#define CHUNK_BYTES ((size_t)256 << 20)   /* workspace per four-step chunk */
+
+static int kerr(int rc) {
+    if (rc) fafi_err("%s", fafk_last_error());
+    return rc;
+}
+
+static long chunk_signals(size_t n_complex, long count) {
+    long c = (long)(CHUNK_BYTES / (n_complex * 2 * sizeof(float)));
+    if (c < 1) c = 1;
+    return c < count ? c : count;
+}
+
+/* ------------------------------------------------------------------ */
+
+int fafi_run_c2c(fafh_plan *p, const float *ir, const float *ii, long idist, float *orr,
+                 float *oi, long odist, long count, void *stream) {
+    const int inter = p->interleaved;
+    switch (p->kind) {
+    case PK_C2C:
+        return kerr(fafk_c2c(&p->a, p->inverse, inter, p->spb, ir, ii, idist, orr, oi, odist,
+                             count, p->scale, stream));
+    case PK_RADER:
+    case PK_BLUE: {
+        /* these kernels read/write split planes; interleaved buffers are
+         * viewed as two strided planes is not supported: deinterleave */
+        if (inter) {
+            fafi_err("Rader/Bluestein lengths need SPLIT layout on the device path");
+            return -1;
+        }
+        if (p->kind == PK_RADER)
+            return kerr(fafk_rader(&p->a, &p->b, &p->rader, p->spb, ir, ii, idist, orr, oi,
+                                   odist, count, p->scale, stream));
+        return kerr(fafk_blue(&p->a, &p->b, &p->blue, p->spb, ir, ii, idist, orr, oi, odist,
+                              count, p->scale, stream));
+    }
+    case PK_C2C_4S: {
+        size_t n = p->n;
+        long cs = chunk_signals(n, count);
+        float *ws = fafi_workspace(p, (size_t)cs * n * 2 * sizeof(float));
+        if (!ws) return -1;
+        float *tr = ws, *ti = ws + (size_t)cs * n;
+        for (long b0 = 0; b0 < count; b0 += cs) {
+            long c = count - b0 < cs ? count - b0 : cs;
+            const long fi = idist * (inter ? 2 : 1), fo = odist * (inter ? 2 : 1);
+            int rc = fafk_fs_col(&p->a, p->inverse, inter ? FAFK_FS_IN_INTER : FAFK_FS_IN_SPLIT,
+                                 p->n2, p->cols, p->twn, ir + b0 * fi,
+                                 inter ? NULL : ii + b0 * fi, fi, tr, ti, (long)n, c, stream);
+            if (!rc)
+                rc = fafk_fs_row(&p->b, p->inverse,
+                                 inter ? FAFK_FS_OUT_INTER : FAFK_FS_OUT_SPLIT, p->n1, p->rows,
+                                 tr, ti, (long)n, orr + b0 * fo, inter ? NULL : oi + b0 * fo,
+                                 fo, c, p->scale, stream);
+            if (kerr(rc)) return -1;
+        }
+        return 0;
+    }
+    default:
+        fafi_err("plan is not a C2C transform");
+        return -1;
+    }
+}
+
+int fafi_run_r2c(fafh_plan *p, const float *x, long idist, float *orr, float *oi, long odist,
+                 long count, void *stream) {
+    const int inter = p->interleaved;
+    if (p->kind == PK_R2C)
+        return kerr(fafk_r2c(&p->a, inter, p->spb, p->post, x, idist, orr, oi, odist, count,
+                             p->scale, stream));
+    if (p->kind != PK_R2C_4S) { fafi_err("plan is not an R2C transform"); return -1; }
+    size_t nc = p->n / 2;
+    long cs = chunk_signals(nc, count);
+    float *ws = fafi_workspace(p, (size_t)cs * nc * 4 * sizeof(float));
+    if (!ws) return -1;
+    float *tr = ws, *ti = ws + (size_t)cs * nc;
+    float *zr = ws + 2 * (size_t)cs * nc, *zi = ws + 3 * (size_t)cs * nc;
+    for (long b0 = 0; b0 < count; b0 += cs) {
+        long c = count - b0 < cs ? count - b0 : cs;
+        long ooff = b0 * odist * (inter ? 2 : 1);
+        /* packed z[i] = x[2i] + i x[2i+1] is read as complex pairs */
+        int rc = fafk_fs_col(&p->a, 0, FAFK_FS_IN_PAIR, p->n2, p->cols, p->twn,
+                             x + b0 * idist, NULL, idist, tr, ti, (long)nc, c, stream);
+        if (!rc)
+            rc = fafk_fs_row(&p->b, 0, FAFK_FS_OUT_SPLIT, p->n1, p->rows, tr, ti, (long)nc, zr,
+                             zi, (long)nc, c, 1.0f, stream);
+        if (!rc)
+            rc = fafk_r2c_post((int)nc, p->post, inter, zr, zi, (long)nc, orr + ooff,
+                               inter ? NULL : oi + ooff, odist, c, p->scale, stream);
+        if (kerr(rc)) return -1;
+    }
+    return 0;
+}
+
+int fafi_run_c2r(fafh_plan *p, const float *ir, const float *ii, long idist, float *y,
+                 long odist, long count, void *stream) {
+    const int inter = p->interleaved;
+    if (p->kind == PK_C2R)
+        return kerr(fafk_c2r(&p->a, inter, p->spb, p->pre, ir, ii, idist, y, odist, count,
+                             p->scale, stream));
+    if (p->kind != PK_C2R_4S) { fafi_err("plan is not a C2R transform"); return -1; }
+    size_t nc = p->n / 2;
+    long cs = chunk_signals(nc, count);
+    float *ws = fafi_workspace(p, (size_t)cs * nc * 4 * sizeof(float));
+    if (!ws) return -1;
+    float *fr = ws, *fi = ws + (size_t)cs * nc;
+    float *tr = ws + 2 * (size_t)cs * nc, *ti = ws + 3 * (size_t)cs * nc;
+    for (long b0 = 0; b0 < count; b0 += cs) {
+        long c = count - b0 < cs ? count - b0 : cs;
+        long ioff = b0 * idist * (inter ? 2 : 1);
+        int rc = fafk_c2r_pre((int)nc, p->pre, inter, ir + ioff, inter ? NULL : ii + ioff,
+                              idist, fr, fi, (long)nc, c, stream);
+        if (!rc)
+            rc = fafk_fs_col(&p->a, 1, FAFK_FS_IN_SPLIT, p->n2, p->cols, p->twn, fr, fi,
+                             (long)nc, tr, ti, (long)nc, c, stream);
+        if (!rc)
+            rc = fafk_fs_row(&p->b, 1, FAFK_FS_OUT_PAIR, p->n1, p->rows, tr, ti, (long)nc,
+                             y + b0 * odist, NULL, odist, c, p->scale, stream);
+        if (kerr(rc)) return -1;
+    }
+    return 0;
+}
+
+int fafi_run_pipe(fafh_plan *p, const float *x, long idist, float *y, long odist, long count,
+                  void *stream) {
+    if (p->kind != PK_PIPE) { fafi_err("plan is not a fused pipeline"); return -1; }
+    if (fafi_pipe_refresh(p, stream)) return -1;
+    fafd_ops ops = p->ops;
+    for (int i = 0; i < ops.nops; ++i)
+        if (ops.code[i] == FAFD_OP_MUL && !ops.h_re) {
+            ops.code[i] = FAFD_OP_NONE;   /* unbound spectrum: no-op, as the reference */
+        }
+    return kerr(fafk_pipe(&p->a, &p->b, p->spb, p->post, p->pre, &ops, x, idist, y, odist,
+                          count, p->scale, stream));
+}

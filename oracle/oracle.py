@@ -1,0 +1,363 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
+leg import this module, and only as the checker / the timed CPU baseline.
+The product library never loads it.
+
+Two libraries:
+  * ``liboracle.so``      -- the C11 restatement in faf_oracle.c (FP64 truth and
+                             the reference's FP32 algorithms);
+  * ``_ref/libfaf_ref.so`` -- the reference library compiled from its own
+                             sources by ``make ref`` (present only where the
+                             reference tree was available at build time).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libfaf_ref.so")
+
+_f = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_sz = C.c_size_t
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        h = C.CDLL(ORACLE_SO)
+        sig = {
+            "orc_is_5_smooth": (C.c_int, [_sz]), "orc_is_7_smooth": (C.c_int, [_sz]),
+            "orc_is_prime": (C.c_int, [_sz]), "orc_rader_eligible": (C.c_int, [_sz]),
+            "orc_primitive_root": (_sz, [_sz]),
+            "orc_factor_7smooth": (C.c_int, [_sz, _i, C.POINTER(C.c_int)]),
+            "orc_next_5_smooth": (_sz, [_sz]), "orc_is_codelet_size": (C.c_int, [_sz]),
+            "orc_fft_size_legal": (C.c_int, [_sz, C.c_int]),
+            "orc_dft_f64": (None, [_sz, C.c_int, _d, _d, _d, _d]),
+            "orc_fft_f64": (C.c_int, [_sz, C.c_int, _d, _d, _d, _d]),
+            "orc_fft_f32": (C.c_int, [_sz, C.c_int, C.c_int, C.c_int, _f, _f, _f, _f]),
+            "orc_rfft_f32": (C.c_int, [_sz, C.c_int, _f, _f, _f]),
+            "orc_irfft_f32": (C.c_int, [_sz, C.c_int, _f, _f, _f]),
+            "orc_herm_mul_f32": (None, [_f, _f, _f, _f, _f, _f, _sz, C.c_int]),
+            "orc_pipeline_f32": (C.c_int, [_sz, C.c_int, _i, _u, _u, C.c_void_p, C.c_void_p,
+                                           _sz, _f, _f]),
+            "orc_hann_f32": (None, [_f, _sz]),
+            "orc_stft_frames": (_sz, [_sz, _sz, _sz]),
+            "orc_stft_f32": (C.c_int, [_sz, _sz, _sz, _f, _f, _f]),
+            "orc_stft_f64": (C.c_int, [_sz, _sz, _sz, _f, _d, _d]),
+            "orc_cwt_real_f32": (C.c_int, [_sz, _sz, _f, _sz, _f, _f]),
+            "orc_cwt_complex_f64": (C.c_int, [_sz, _sz, _f, _sz, _f, _d, _d]),
+            "orc_time_rfft_batch": (C.c_double, [_sz, _sz, C.c_int, _f, _f, _f]),
+        }
+        for k, (r, a) in sig.items():
+            fn = getattr(h, k)
+            fn.restype = r
+            fn.argtypes = a
+        _lib = h
+    return _lib
+
+
+# ---------------------------------------------------------------- FP64 truth
+
+def dft64(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    """Direct DFT with exact (j*k mod n) phase indexing; x complex or real."""
+    x = np.asarray(x)
+    n = x.shape[-1]
+    xr = np.ascontiguousarray(x.real, dtype=np.float64)
+    xi = np.ascontiguousarray(np.imag(x), dtype=np.float64)
+    yr = np.empty(n)
+    yi = np.empty(n)
+    lib().orc_dft_f64(n, int(inverse), xr, xi, yr, yi)
+    return yr + 1j * yi
+
+
+def fft64(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    """FP64 mixed-radix FFT (exact-index twiddle table) along the last axis."""
+    x = np.atleast_2d(np.asarray(x))
+    out = np.empty(x.shape, dtype=np.complex128)
+    n = x.shape[-1]
+    for r in range(x.shape[0]):
+        xr = np.ascontiguousarray(x[r].real, dtype=np.float64)
+        xi = np.ascontiguousarray(np.imag(x[r]), dtype=np.float64)
+        yr = np.empty(n)
+        yi = np.empty(n)
+        lib().orc_fft_f64(n, int(inverse), xr, xi, yr, yi)
+        out[r] = yr + 1j * yi
+    return out
+
+
+# ---------------------------------------------- FP32 restatement of the reference
+
+NORM = {"none": 1, "ortho": 2, "forward": 3}
+
+
+def fft32(x: np.ndarray, inverse=False, norm="none", bluestein=False) -> np.ndarray:
+    x = np.atleast_2d(np.asarray(x, dtype=np.complex64))
+    n = x.shape[-1]
+    out = np.empty(x.shape, dtype=np.complex64)
+    for r in range(x.shape[0]):
+        xr = np.ascontiguousarray(x[r].real, dtype=np.float32)
+        xi = np.ascontiguousarray(x[r].imag, dtype=np.float32)
+        yr = np.empty(n, np.float32)
+        yi = np.empty(n, np.float32)
+        rc = lib().orc_fft_f32(n, int(inverse), NORM[norm], int(bluestein), xr, xi, yr, yi)
+        if rc:
+            raise ValueError(f"oracle: illegal FFT length {n}")
+        out[r] = yr + 1j * yi
+    return out
+
+
+def rfft32(x: np.ndarray, norm="none") -> np.ndarray:
+    x = np.atleast_2d(np.asarray(x, dtype=np.float32))
+    n = x.shape[-1]
+    nb = n // 2 + 1
+    out = np.empty((x.shape[0], nb), dtype=np.complex64)
+    for r in range(x.shape[0]):
+        re = np.empty(nb, np.float32)
+        im = np.empty(nb, np.float32)
+        if lib().orc_rfft_f32(n, NORM[norm], np.ascontiguousarray(x[r]), re, im):
+            raise ValueError(f"oracle: illegal rfft length {n}")
+        out[r] = re + 1j * im
+    return out
+
+
+def irfft32(X: np.ndarray, n: int, norm="none") -> np.ndarray:
+    X = np.atleast_2d(np.asarray(X, dtype=np.complex64))
+    out = np.empty((X.shape[0], n), dtype=np.float32)
+    for r in range(X.shape[0]):
+        y = np.empty(n, np.float32)
+        re = np.ascontiguousarray(X[r].real, dtype=np.float32)
+        im = np.ascontiguousarray(X[r].imag, dtype=np.float32)
+        if lib().orc_irfft_f32(n, NORM[norm], re, im, y):
+            raise ValueError(f"oracle: illegal irfft length {n}")
+        out[r] = y
+    return out
+
+
+def pipeline32(x, n, ops, a1=None, a2=None, h=None):
+    """ops: list of 1 (conj), 2 (mul by bound h), 3 (bandpass a1..a2)."""
+    x = np.atleast_2d(np.asarray(x, dtype=np.float32))
+    k = len(ops)
+    o = np.asarray(ops, dtype=np.int32) if k else np.zeros(1, np.int32)
+    b1 = np.asarray(a1 if a1 is not None else [0] * max(k, 1), dtype=np.uint32)
+    b2 = np.asarray(a2 if a2 is not None else [0] * max(k, 1), dtype=np.uint32)
+    hr = hi = None
+    hb = 0
+    if h is not None:
+        hr = np.ascontiguousarray(np.real(h), dtype=np.float32)
+        hi = np.ascontiguousarray(np.imag(h), dtype=np.float32)
+        hb = hr.size
+    out = np.empty_like(x)
+    for r in range(x.shape[0]):
+        y = np.empty(n, np.float32)
+        lib().orc_pipeline_f32(n, k, o, b1, b2,
+                               hr.ctypes.data if hr is not None else None,
+                               hi.ctypes.data if hi is not None else None, hb,
+                               np.ascontiguousarray(x[r]), y)
+        out[r] = y
+    return out
+
+
+def hann32(n: int) -> np.ndarray:
+    w = np.empty(n, np.float32)
+    lib().orc_hann_f32(w, n)
+    return w
+
+
+def stft_frames(length, win, hop):
+    return int(lib().orc_stft_frames(length, win, hop))
+
+
+def stft32(x, win, hop):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    nf = stft_frames(x.size, win, hop)
+    nb = win // 2 + 1
+    re = np.empty(nf * nb, np.float32)
+    im = np.empty(nf * nb, np.float32)
+    lib().orc_stft_f32(x.size, win, hop, x, re, im)
+    return (re + 1j * im).reshape(nf, nb)
+
+
+def stft64(x, win, hop):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    nf = stft_frames(x.size, win, hop)
+    nb = win // 2 + 1
+    re = np.empty(nf * nb)
+    im = np.empty(nf * nb)
+    lib().orc_stft_f64(x.size, win, hop, x, re, im)
+    return (re + 1j * im).reshape(nf, nb)
+
+
+def cwt_real32(x, psi, stride):
+    """psi: float32 [rows, stride] live table."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n = x.size
+    rows = psi.shape[0]
+    w = np.empty(rows * n, np.float32)
+    lib().orc_cwt_real_f32(n, rows, np.ascontiguousarray(psi, dtype=np.float32), stride, x, w)
+    return w.reshape(rows, n)
+
+
+def cwt_complex64(x, psi, stride):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n = x.size
+    rows = psi.shape[0]
+    zr = np.empty(rows * n)
+    zi = np.empty(rows * n)
+    lib().orc_cwt_complex_f64(n, rows, np.ascontiguousarray(psi, dtype=np.float32), stride,
+                              x, zr, zi)
+    return (zr + 1j * zi).reshape(rows, n)
+
+
+def time_rfft_batch(x: np.ndarray, threads: int):
+    """CPU baseline: seconds for rfft of every row of x on `threads` threads."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    cnt, n = x.shape
+    nb = n // 2 + 1
+    re = np.empty(cnt * nb, np.float32)
+    im = np.empty(cnt * nb, np.float32)
+    dt = lib().orc_time_rfft_batch(n, cnt, threads, x.ravel(), re, im)
+    return dt, (re + 1j * im).reshape(cnt, nb)
+
+
+# --------------------------------------------------- the compiled reference
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+_ref = None
+
+
+def ref():
+    """The reference library compiled from its own sources (oracle/_ref)."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise FileNotFoundError(REF_SO)
+        import fastandfourier_amd as fa   # struct layouts only; the reference ABI is the same
+        h = C.CDLL(REF_SO)
+        PT = C.c_void_p
+        sig = {
+            "faf_create_fft": (PT, [C.POINTER(fa.FafConfig)]),
+            "faf_create_rfft": (PT, [C.POINTER(fa.FafConfig)]),
+            "faf_create_inverse": (PT, [PT]),
+            "faf_execute": (C.c_int, [PT, C.POINTER(fa.FafBuffer), C.POINTER(fa.FafBuffer)]),
+            "faf_destroy_transform": (None, [PT]),
+            "faf_get_error": (C.c_char_p, []),
+            "faf_is_size_supported": (C.c_bool, [C.c_int, _sz]),
+            "faf_get_recommended_size": (_sz, [C.c_int, _sz]),
+            "faf_cwt_config_init": (fa.FafCwtConfig, [_sz]),
+            "faf_create_cwt": (PT, [C.POINTER(fa.FafCwtConfig)]),
+            "faf_cwt_n_rows": (_sz, [PT]), "faf_cwt_n_scales": (_sz, [PT]),
+            "faf_cwt_n_bins": (_sz, [PT]),
+            "faf_cwt_psi_f32": (C.POINTER(C.c_float), [PT, _sz]),
+            "faf_cwt_phi_f32": (C.POINTER(C.c_float), [PT]),
+            "faf_cwt_bank_report": (C.POINTER(fa.FafLpReport), [PT]),
+            "faf_cwt_freqs": (C.c_int, [PT, C.POINTER(C.c_double), _sz]),
+            "faf_cwt_scales": (C.c_int, [PT, C.POINTER(C.c_double), _sz]),
+            "chirp_compile": (PT, [C.c_char_p]),
+            "chirp_bind": (C.c_int, [PT, C.c_char_p, C.c_void_p, C.c_void_p, _sz]),
+            "chirp_cleanup": (None, []),
+        }
+        for k, (r, a) in sig.items():
+            fn = getattr(h, k)
+            fn.restype = r
+            fn.argtypes = a
+        _ref = h
+    return _ref
+
+
+def ref_fft(x: np.ndarray, inverse=False, norm="none", bluestein=False) -> np.ndarray:
+    """C2C through the compiled reference (SPLIT layout, FP32, VM backend)."""
+    import fastandfourier_amd as fa
+    x = np.atleast_2d(np.asarray(x, dtype=np.complex64))
+    n = x.shape[-1]
+    h = ref()
+    cfg = fa.config_init(n, layout="split", norm=norm, dir="inverse" if inverse else "forward",
+                         backend=1, flags=fa.FLAG_BLUESTEIN if bluestein else 0)
+    t = h.faf_create_fft(C.byref(cfg))
+    if not t:
+        raise ValueError(h.faf_get_error().decode())
+    out = np.empty_like(x)
+    try:
+        for r in range(x.shape[0]):
+            xr = np.ascontiguousarray(x[r].real)
+            xi = np.ascontiguousarray(x[r].imag)
+            yr = np.zeros(n, np.float32)
+            yi = np.zeros(n, np.float32)
+            bi = fa.buffer_split(xr.ctypes.data, xi.ctypes.data, n)
+            bo = fa.buffer_split(yr.ctypes.data, yi.ctypes.data, n)
+            if h.faf_execute(t, C.byref(bo), C.byref(bi)):
+                raise RuntimeError(h.faf_get_error().decode())
+            out[r] = yr + 1j * yi
+    finally:
+        h.faf_destroy_transform(t)
+    return out
+
+
+def ref_rfft(x: np.ndarray, norm="none") -> np.ndarray:
+    import fastandfourier_amd as fa
+    x = np.atleast_2d(np.asarray(x, dtype=np.float32))
+    n = x.shape[-1]
+    nb = n // 2 + 1
+    h = ref()
+    cfg = fa.config_init(n, norm=norm, backend=1)
+    t = h.faf_create_rfft(C.byref(cfg))
+    if not t:
+        raise ValueError(h.faf_get_error().decode())
+    out = np.empty((x.shape[0], nb), np.complex64)
+    try:
+        for r in range(x.shape[0]):
+            xr = np.ascontiguousarray(x[r])
+            re = np.zeros(nb, np.float32)
+            im = np.zeros(nb, np.float32)
+            bi = fa.buffer_real(xr.ctypes.data, n)
+            bo = fa.buffer_hermitian(re.ctypes.data, im.ctypes.data, nb)
+            if h.faf_execute(t, C.byref(bo), C.byref(bi)):
+                raise RuntimeError(h.faf_get_error().decode())
+            out[r] = re + 1j * im
+    finally:
+        h.faf_destroy_transform(t)
+    return out
+
+
+def ref_irfft(X: np.ndarray, n: int, norm="none") -> np.ndarray:
+    import fastandfourier_amd as fa
+    X = np.atleast_2d(np.asarray(X, dtype=np.complex64))
+    nb = n // 2 + 1
+    h = ref()
+    cfg = fa.config_init(n, norm=norm, dir="inverse", backend=1)
+    t = h.faf_create_rfft(C.byref(cfg))
+    if not t:
+        raise ValueError(h.faf_get_error().decode())
+    out = np.empty((X.shape[0], n), np.float32)
+    try:
+        for r in range(X.shape[0]):
+            re = np.ascontiguousarray(X[r].real)
+            im = np.ascontiguousarray(X[r].imag)
+            y = np.zeros(n, np.float32)
+            bi = fa.buffer_hermitian(re.ctypes.data, im.ctypes.data, nb)
+            bo = fa.buffer_real(y.ctypes.data, n)
+            if h.faf_execute(t, C.byref(bo), C.byref(bi)):
+                raise RuntimeError(h.faf_get_error().decode())
+            out[r] = y
+    finally:
+        h.faf_destroy_transform(t)
+    return out
