@@ -1,0 +1,318 @@
+"""Benchmark of the transform execution path on one MI355X.
+
+Headline (the workload BASELINE names): batched R2C rfft, N=4096, FP32 real
+input N(0,1), batch 2^28/N = 65536 signals (1 GiB in), Hermitian-packed
+N/2+1 bins out.  A step is one batched rfft launch over the whole batch with
+inputs resident in HBM.  value = algorithmic bytes per step / step time,
+algorithmic bytes = batch * (4N + 8(N/2+1)).
+
+Also reported (the "extra" object): C2R and the other N of the same config,
+the fused notch pipeline, C2C N=4096 batch 256, the ragged persistent launch,
+framed STFT, the CWT bank, a device-to-device copy ceiling, and the plain
+per-call rate of the same kernels.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--only", default="", help="comma list of extra workloads")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU: each box has one MI355X; "
+                                   "the batch shards across GPUs with no exchange "
+                                   "(see DESIGN.md), run with --gpus 1"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import fastandfourier_amd as fa
+
+    lib = fa.load()
+    fa.check(lib.faf_amd_device_ready(), "device")
+    log("device ready")
+    stream = torch.cuda.current_stream()
+    lib.faf_amd_set_stream(C.c_void_p(stream.cuda_stream))
+    lib.faf_amd_set_sync(0)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps          # ms per step
+
+    extra = {}
+
+    # ---------------- headline: batched rfft N=4096, 1 GiB in ----------------
+    n = 4096
+    b = (1 << 28) // n
+    nb = n // 2 + 1
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.randn((b, n), generator=gen, device="cuda", dtype=torch.float32)
+    re = torch.empty((b, nb), device="cuda")
+    im = torch.empty((b, nb), device="cuda")
+    y = torch.empty((b, n), device="cuda")
+    t = lib.faf_create_rfft(C.byref(fa.config_init(n)))
+    ti = lib.faf_create_inverse(t)
+    assert t and ti, lib.faf_get_error()
+    bi = fa.buffer_real(x.data_ptr(), n)
+    bo = fa.buffer_hermitian(re.data_ptr(), im.data_ptr(), nb)
+    by = fa.buffer_real(y.data_ptr(), n)
+
+    def step_r2c():
+        fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0), "rfft")
+
+    def step_c2r():
+        fa.check(lib.faf_execute_batch(ti, C.byref(by), C.byref(bo), b, 0, 0), "irfft")
+
+    algo_bytes = b * (4 * n + 8 * nb)
+    log("headline rfft N=4096 batch", b)
+    ms = timed(step_r2c, args.steps, args.warmup)
+    gbs = algo_bytes / (ms * 1e-3) / 1e9
+    ms_inv = timed(step_c2r, args.steps, args.warmup)
+    extra["c2r_4096_gbs"] = round(algo_bytes / (ms_inv * 1e-3) / 1e9, 1)
+    torch.cuda.synchronize()
+    rt = (torch.linalg.vector_norm(y - x) / torch.linalg.vector_norm(x)).item()
+    extra["c2_4096_round_trip_rel_l2"] = rt
+
+    # ---------------- CPU baseline: the oracle's restatement of the reference's
+    # CPU rfft on this host's cores, bounded sample of the same signals -------
+    cpu = None
+    log("cpu baseline")
+    try:
+        import oracle as O
+        threads = len(os.sched_getaffinity(0))
+        sample = min(b, 16384)
+        xs = x[:sample].cpu().numpy()
+        O.time_rfft_batch(xs[:256], threads)                   # warm
+        dts = [O.time_rfft_batch(xs, threads)[0] for _ in range(3)]
+        dt = sorted(dts)[1]
+        cpu = {"value": round(sample * (4 * n + 8 * nb) / dt / 1e9, 3), "unit": "GB/s",
+               "cores": threads, "kind": "port",
+               "sample": f"{sample} of the {b} N=4096 signals (same bytes), median of 3, "
+                         f"{dt:.2f} s wall on {threads} threads"}
+        extra["cpu_sample_rel_l2_vs_gpu"] = None
+    except Exception as e:                                     # report, do not hide
+        cpu = {"value": None, "unit": "GB/s", "cores": None, "kind": "port",
+               "sample": f"unavailable: {e}"}
+
+    roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile()}
+
+    # plain per-call launches of the same kernel (one faf_execute per signal)
+    def plain(count):
+        x1 = fa.buffer_real(x.data_ptr(), n)
+        o1 = fa.buffer_hermitian(re.data_ptr(), im.data_ptr(), nb)
+        for _ in range(count):
+            fa.check(lib.faf_execute_batch(t, C.byref(o1), C.byref(x1), 1, 0, 0), "rfft1")
+    pc = 2000
+    ms_plain = timed(lambda: plain(pc), 1, 1)
+    extra["plain_per_call_rfft_4096_gbs"] = round(pc * (4 * n + 8 * nb) / (ms_plain * 1e-3) / 1e9, 2)
+    extra["batched_over_plain"] = round(gbs / extra["plain_per_call_rfft_4096_gbs"], 1)
+
+    # copy ceiling on this box (read + write)
+    z = torch.empty_like(x)
+    ms_copy = timed(lambda: z.copy_(x), 10, 2)
+    extra["d2d_copy_gbs"] = round(2 * x.numel() * 4 / (ms_copy * 1e-3) / 1e9, 1)
+    del z
+    lib.faf_destroy_transform(t)
+    lib.faf_destroy_transform(ti)
+    del x, re, im, y
+    torch.cuda.empty_cache()
+
+    if not args.quick:
+        only = set(filter(None, args.only.split(",")))
+        run_extras(lib, fa, torch, np, timed, extra, only)
+
+    line = {
+        "metric": "batched FFT GB/s (% HBM peak) vs plain launch; p50/p99 at mixed-size load",
+        "value": round(gbs, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic N(0,1) float32 (torch CUDA generator, seed 0), resident in HBM",
+        "config": {"workload": "batched R2C rfft N=4096, batch 65536 (1 GiB in), Hermitian "
+                               "N/2+1 bins, NumPy norm", "n": n, "batch": b},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line))
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the headline kernel, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy
+    kernel of known size), or None when no such summary exists."""
+    p = os.path.join(ROOT, "profiles", "k_r2c_rfft4096_pmc.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return int(float(line.split(":", 1)[1]))
+            except ValueError:
+                return None
+    return None
+
+
+def run_extras(lib, fa, torch, np, timed, extra, only):
+    def want(k):
+        ok = not only or k in only
+        if ok:
+            log("extra", k)
+        return ok
+
+    # rfft / irfft at the other N of the config
+    if want("c2"):
+        for n in (1024, 16384, 65536):
+            b = (1 << 28) // n
+            nb = n // 2 + 1
+            x = torch.randn((b, n), device="cuda")
+            re = torch.empty((b, nb), device="cuda")
+            im = torch.empty((b, nb), device="cuda")
+            y = torch.empty((b, n), device="cuda")
+            t = lib.faf_create_rfft(C.byref(fa.config_init(n)))
+            ti = lib.faf_create_inverse(t)
+            bi = fa.buffer_real(x.data_ptr(), n)
+            bo = fa.buffer_hermitian(re.data_ptr(), im.data_ptr(), nb)
+            by = fa.buffer_real(y.data_ptr(), n)
+            ab = b * (4 * n + 8 * nb)
+            ms = timed(lambda: fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0)), 5, 1)
+            ms2 = timed(lambda: fa.check(lib.faf_execute_batch(ti, C.byref(by), C.byref(bo), b, 0, 0)), 5, 1)
+            extra[f"r2c_{n}_gbs"] = round(ab / (ms * 1e-3) / 1e9, 1)
+            extra[f"c2r_{n}_gbs"] = round(ab / (ms2 * 1e-3) / 1e9, 1)
+            lib.faf_destroy_transform(t)
+            lib.faf_destroy_transform(ti)
+            del x, re, im, y
+            torch.cuda.empty_cache()
+
+    # fused notch pipeline N=4096, 1 GiB
+    if want("notch"):
+        n = 4096
+        b = (1 << 28) // n
+        t = lib.chirp_compile(b"(pipeline (rfft :size 4096) (mul-spectrum) (irfft))")
+        nb = n // 2 + 1
+        m = np.ones(nb, np.float32)
+        m[-(-n // 10): n // 8] = 0
+        hr = torch.from_numpy(m).cuda()
+        hi = torch.zeros_like(hr)
+        lib.chirp_bind(t, b"H", hr.data_ptr(), hi.data_ptr(), nb)
+        x = torch.randn((b, n), device="cuda")
+        y = torch.empty_like(x)
+        bi = fa.buffer_real(x.data_ptr(), n)
+        bo = fa.buffer_real(y.data_ptr(), n)
+        ms = timed(lambda: fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0)), 5, 1)
+        extra["notch_pipeline_4096_gbs"] = round(b * 8 * n / (ms * 1e-3) / 1e9, 1)
+        lib.faf_destroy_transform(t)
+        del x, y
+        torch.cuda.empty_cache()
+
+    # C2C N=4096 batch 256 (C1): launch-bound, report time too
+    if want("c1"):
+        n, b = 4096, 256
+        xr = torch.rand((b, n), device="cuda") * 2 - 1
+        xi = torch.rand((b, n), device="cuda") * 2 - 1
+        yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+        t = lib.faf_create_fft(C.byref(fa.config_init(n)))
+        bi = fa.buffer_split(xr.data_ptr(), xi.data_ptr(), n)
+        bo = fa.buffer_split(yr.data_ptr(), yi.data_ptr(), n)
+        ms = timed(lambda: fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0)), 50, 5)
+        extra["c1_c2c_4096x256_us"] = round(ms * 1e3, 2)
+        extra["c1_c2c_4096x256_gbs"] = round(b * 16 * n / (ms * 1e-3) / 1e9, 1)
+        lib.faf_destroy_transform(t)
+
+    # C3 ragged: 20,000 signals, one persistent launch
+    if want("c3"):
+        lens_set = [60, 360, 1000, 3000, 4800, 11, 17, 31, 97, 8192, 1009]
+        rng = np.random.default_rng(0)
+        lens = rng.choice(lens_set, size=20000)
+        arr = (C.c_size_t * len(lens))(*[int(v) for v in lens])
+        r = lib.faf_ragged_create(arr, len(lens), 0)
+        tot = lib.faf_ragged_total(r)
+        xr = torch.rand(tot, device="cuda") * 2 - 1
+        xi = torch.rand(tot, device="cuda") * 2 - 1
+        yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+        ms = timed(lambda: fa.check(lib.faf_ragged_execute(r, yr.data_ptr(), yi.data_ptr(), xr.data_ptr(), xi.data_ptr())), 10, 2)
+        extra["c3_ragged_20000_ms"] = round(ms, 3)
+        extra["c3_ragged_gbs"] = round(16 * tot / (ms * 1e-3) / 1e9, 1)
+        lib.faf_ragged_destroy(r)
+
+    # framed STFT: 64 channels x 2^22, Hann 1024, hop 256
+    if want("stft"):
+        win, hop, L, ch = 1024, 256, 1 << 22, 64
+        cfg = fa.config_init(win, layout="real", win_length=win, hop_length=hop)
+        t = lib.faf_create_stft(C.byref(cfg))
+        nf = lib.faf_stft_frames(t, L)
+        nb = win // 2 + 1
+        x = torch.randn((ch, L), device="cuda")
+        re = torch.empty((ch, nf, nb), device="cuda")
+        im = torch.empty((ch, nf, nb), device="cuda")
+        ms = timed(lambda: fa.check(lib.faf_execute_stft(t, re.data_ptr(), im.data_ptr(), x.data_ptr(), L, ch, 0)), 3, 1)
+        extra["stft_64x2^22_ms"] = round(ms, 3)
+        extra["stft_gbs"] = round((4 * L * ch + 8 * nb * nf * ch) / (ms * 1e-3) / 1e9, 1)
+        lib.faf_destroy_transform(t)
+        del x, re, im
+        torch.cuda.empty_cache()
+
+    # CWT bank: 4096-sample segments, 64 scales + lowpass, complex64 rows
+    if want("cwt"):
+        n = 4096
+        cc = lib.faf_cwt_config_init(n)
+        cc.fs = 16000.0
+        cc.voices = 11
+        cc.fmin = 62.5
+        cc.fmax = 62.5 * 2 ** (63.5 / 11)
+        t = lib.faf_create_cwt(C.byref(cc))
+        rows = lib.faf_cwt_n_rows(t)
+        segs = 1024
+        x = torch.randn((segs, n), device="cuda")
+        z = torch.empty((segs, rows, n, 2), device="cuda")
+        ms = timed(lambda: fa.check(lib.faf_execute_cwt_complex(t, z.data_ptr(), x.data_ptr(), segs, 0)), 3, 1)
+        extra["cwt_1024seg_ms"] = round(ms, 3)
+        extra["cwt_gbs"] = round(segs * (4 * n + 8 * n * rows) / (ms * 1e-3) / 1e9, 1)
+        lib.faf_destroy_transform(t)
+        del x, z
+        torch.cuda.empty_cache()
+
+
+if __name__ == "__main__":
+    main()

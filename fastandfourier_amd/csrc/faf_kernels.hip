@@ -27,10 +27,16 @@ static int fail(const char *what, hipError_t e) {
 
 static inline hipStream_t S(void *s) { return (hipStream_t)s; }
 
-/* allow every kernel the full 160 KB of dynamic LDS */
+/* allow every kernel the LDS left beside its static LDS (160 KB in all);
+ * a refusal is not fatal (launches then stay within the default), and the
+ * error it records is cleared so it cannot surface in a later call */
 template <class K> static void lds_attr(K k) {
+    hipFuncAttributes fa;
+    size_t stat = 0;
+    if (hipFuncGetAttributes(&fa, (const void *)k) == hipSuccess) stat = fa.sharedSizeBytes;
     (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              FAFD_MAX_LDS_BYTES);
+                              (int)(FAFD_MAX_LDS_BYTES - stat));
+    (void)hipGetLastError();
 }
 
 extern "C" int fafk_init(void) {

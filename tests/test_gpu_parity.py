@@ -1,0 +1,574 @@
+"""GPU parity tests: the gfx950 path through the C ABI against the oracle.
+
+Bars (stated per the transform path's contract):
+  * against FP64 ground truth: relative L2 <= 1e-6 * log2(N);
+  * against the reference library's own FP32 output (tests/golden): the same
+    bar (both sides are FP32 computations of the same transform);
+  * integer/layout behaviour (error codes, Hermitian DC/Nyquist imag = 0,
+    batch membership) exactly.
+Inputs: numpy PCG64, uniform[-1, 1) complex / N(0, 1) real, as float32.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import fastandfourier_amd as fa
+import oracle as O
+from conftest import rel_l2, tol
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import gpu_util as G  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def cplx(rng, k, n):
+    return (rng.uniform(-1, 1, (k, n)) + 1j * rng.uniform(-1, 1, (k, n))).astype(np.complex64)
+
+
+def truth_fft(x, inverse=False):
+    y = O.fft64(x.astype(np.complex128), inverse=inverse)
+    return y
+
+
+def test_device_ready():
+    say("case: device ready")
+    assert torch.cuda.is_available()
+    fa.check(G.L().faf_amd_device_ready(), "device")
+
+
+# ------------------------------------------------------------------ C2C
+
+C2C_SIZES = [2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 16, 32, 60, 64, 128, 256, 360, 512, 1000, 1024,
+             2048, 3000, 3840, 4000, 4050, 4096, 4800, 8192, 11, 13, 17, 19, 29, 31, 37, 97, 1009,
+             16384, 20000, 32768, 65536]
+
+
+@pytest.mark.parametrize("n", C2C_SIZES)
+def test_c2c_matches_fp64(n):
+    say(f"case: c2c n={n}")
+    rng = np.random.default_rng(n)
+    x = cplx(rng, 3, n)
+    t = G.create("fft", n)
+    ti = G.create("fft", n, dir="inverse")
+    try:
+        y = G.c2c(t, x)
+        assert rel_l2(y, truth_fft(x)) < tol(n)
+        z = G.c2c(ti, y)
+        assert rel_l2(z, x) < 2 * tol(n)                     # NumPy norm: round trip
+    finally:
+        G.destroy(t)
+        G.destroy(ti)
+
+
+def test_c1_batch256_n4096_forward_inverse():
+    """Config C1: N=4096, batch 256, uniform[-1,1] from seed 0, NumPy norm."""
+    say("case: C1 c2c 4096 x 256")
+    n, b = 4096, 256
+    rng = np.random.default_rng(0)
+    x = cplx(rng, b, n)
+    t = G.create("fft", n)
+    ti = G.create("fft", n, dir="inverse")
+    try:
+        y = G.c2c(t, x)
+        assert rel_l2(y, truth_fft(x)) < tol(n)
+        assert rel_l2(y[:4], O.fft32(x[:4])) < tol(n)        # beside the reference's CPU path
+        z = G.c2c(ti, y)
+        assert rel_l2(z, truth_fft(y.astype(np.complex128), inverse=True) / n) < tol(n)
+        assert rel_l2(z, x) < 2 * tol(n)
+    finally:
+        G.destroy(t)
+        G.destroy(ti)
+
+
+@pytest.mark.parametrize("norm", ["ortho", "forward"])
+def test_c2c_norms(norm):
+    say(f"case: c2c norm {norm}")
+    n = 1024
+    rng = np.random.default_rng(3)
+    x = cplx(rng, 2, n)
+    t = G.create("fft", n, norm=norm)
+    ti = G.create("fft", n, norm=norm, dir="inverse")
+    try:
+        s = 1 / np.sqrt(n) if norm == "ortho" else 1 / n
+        assert rel_l2(G.c2c(t, x), truth_fft(x) * s) < tol(n)
+        si = 1 / np.sqrt(n) if norm == "ortho" else 1.0
+        assert rel_l2(G.c2c(ti, x), truth_fft(x, inverse=True) * si) < tol(n)
+    finally:
+        G.destroy(t)
+        G.destroy(ti)
+
+
+@pytest.mark.parametrize("n", [64, 4096, 65536])
+def test_c2c_interleaved(n):
+    say(f"case: c2c interleaved n={n}")
+    rng = np.random.default_rng(5)
+    x = cplx(rng, 2, n)
+    t = G.create("fft", n, layout="interleaved")
+    try:
+        assert rel_l2(G.c2c(t, x, inter=True), truth_fft(x)) < tol(n)
+    finally:
+        G.destroy(t)
+
+
+def test_c2c_golden_reference_outputs(golden):
+    """Against the reference library's FP32 outputs (tests/golden)."""
+    for key in golden.files:
+        if not (key.startswith("c2c_") and key.endswith("_in")):
+            continue
+        n = int(key.split("_")[1])
+        say(f"case: golden c2c n={n}")
+        x = golden[key]
+        t = G.create("fft", n)
+        ti = G.create("fft", n, dir="inverse")
+        try:
+            assert rel_l2(G.c2c(t, x), golden[f"c2c_{n}_fwd"]) < tol(n), n
+            assert rel_l2(G.c2c(ti, x), golden[f"c2c_{n}_inv"]) < tol(n), n
+        finally:
+            G.destroy(t)
+            G.destroy(ti)
+
+
+@pytest.mark.parametrize("n", [23, 47, 307, 4097])
+def test_bluestein(n, golden):
+    say(f"case: bluestein n={n}")
+    rng = np.random.default_rng(n)
+    x = cplx(rng, 2, n)
+    with pytest.raises(fa.FafError):
+        G.create("fft", n)                                       # opt-in only
+    t = G.create("fft", n, flags=fa.FLAG_BLUESTEIN)
+    ti = G.create("fft", n, flags=fa.FLAG_BLUESTEIN, dir="inverse")
+    try:
+        assert rel_l2(G.c2c(t, x), truth_fft(x)) < tol(n)
+        assert rel_l2(G.c2c(ti, x), truth_fft(x, inverse=True) / n) < tol(n)
+        if f"blue_{n}_in" in golden.files:
+            assert rel_l2(G.c2c(t, golden[f"blue_{n}_in"]), golden[f"blue_{n}_fwd"]) < 2 * tol(n)
+    finally:
+        G.destroy(t)
+        G.destroy(ti)
+
+
+def test_host_pointers_are_staged():
+    """The drop-in path: host numpy buffers through faf_execute."""
+    say("case: host pointers")
+    lib = G.L()
+    n = 360
+    rng = np.random.default_rng(9)
+    x = cplx(rng, 1, n)[0]
+    xr, xi = np.ascontiguousarray(x.real), np.ascontiguousarray(x.imag)
+    yr, yi = np.zeros(n, np.float32), np.zeros(n, np.float32)
+    t = G.create("fft", n)
+    try:
+        bi = fa.buffer_split(xr.ctypes.data, xi.ctypes.data, n)
+        bo = fa.buffer_split(yr.ctypes.data, yi.ctypes.data, n)
+        fa.check(lib.faf_execute(t, C.byref(bo), C.byref(bi)), "execute")
+        assert rel_l2(yr + 1j * yi, truth_fft(x)[0]) < tol(n)
+        # in place on host memory
+        fa.check(lib.faf_execute(t, C.byref(bi), C.byref(bi)), "execute in place")
+        assert rel_l2(xr + 1j * xi, truth_fft(x)[0]) < tol(n)
+    finally:
+        G.destroy(t)
+
+
+def test_error_paths():
+    say("case: error paths")
+    lib = G.L()
+    t = G.create("fft", 64)
+    try:
+        a = G.empty((2, 64))
+        bi = fa.buffer_interleaved(a.data_ptr(), 64)
+        bo = fa.buffer_split(a[0].data_ptr(), a[1].data_ptr(), 64)
+        assert lib.faf_execute(t, C.byref(bo), C.byref(bi)) == -1
+        assert b"layout" in lib.faf_get_error()
+    finally:
+        G.destroy(t)
+    r = G.create("rfft", 64)
+    try:
+        a = G.empty((3, 64))
+        bi = fa.buffer_real(a[0].data_ptr(), 63)
+        bo = fa.buffer_hermitian(a[1].data_ptr(), a[2].data_ptr(), 33)
+        assert lib.faf_execute(r, C.byref(bo), C.byref(bi)) == -1
+        assert b"expected 64" in lib.faf_get_error()
+    finally:
+        G.destroy(r)
+    for bad in (14, 23, 4097):
+        with pytest.raises(fa.FafError):
+            G.create("fft", bad)
+    with pytest.raises(fa.FafError):
+        G.create("rfft", 15)
+
+
+# ------------------------------------------------------------------ R2C / C2R
+
+@pytest.mark.parametrize("n", [4, 8, 30, 64, 1000, 1024, 4096, 16384, 65536])
+def test_rfft_irfft(n):
+    say(f"case: rfft n={n}")
+    rng = np.random.default_rng(n + 1)
+    x = rng.standard_normal((3, n)).astype(np.float32)
+    t = G.create("rfft", n)
+    ti = G.create("rfft", n, dir="inverse")
+    try:
+        X = G.r2c(t, x)
+        truth = np.stack([O.fft64(r.astype(np.float64))[0][: n // 2 + 1] for r in x])
+        assert rel_l2(X, truth) < tol(n)
+        assert np.all(X[:, 0].imag == 0) and np.all(X[:, -1].imag == 0)
+        y = G.c2r(ti, X, n)
+        assert rel_l2(y, x) < 2 * tol(n)
+    finally:
+        G.destroy(t)
+        G.destroy(ti)
+
+
+def test_rfft_golden_reference_outputs(golden):
+    for n in (1024, 4096, 16384):
+        say(f"case: golden rfft n={n}")
+        x = golden[f"rfft_{n}_in"]
+        t = G.create("rfft", n)
+        to = G.create("rfft", n, norm="ortho")
+        ti = G.create("rfft", n, dir="inverse")
+        try:
+            assert rel_l2(G.r2c(t, x), golden[f"rfft_{n}_fwd"]) < tol(n)
+            assert rel_l2(G.r2c(to, x), golden[f"rfft_{n}_ortho"]) < tol(n)
+            assert rel_l2(G.c2r(ti, golden[f"rfft_{n}_fwd"], n), golden[f"rfft_{n}_inv"]) < tol(n)
+        finally:
+            G.destroy(t)
+            G.destroy(to)
+            G.destroy(ti)
+
+
+@pytest.mark.parametrize("n", [1024, 4096, 16384, 65536])
+def test_c2_full_batch_round_trip(n):
+    """Config C2 at its full size: batch = 2^28 / N signals (1 GiB in).
+
+    Size-independent properties: irfft(rfft(x)) == x, Hermitian DC/Nyquist
+    imag exactly 0, and bins of a sample of rows against FP64."""
+    say(f"case: C2 full batch n={n}")
+    b = (1 << 28) // n
+    nb = n // 2 + 1
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.randn((b, n), generator=gen, device="cuda", dtype=torch.float32)
+    re = torch.empty((b, nb), device="cuda")
+    im = torch.empty((b, nb), device="cuda")
+    y = torch.empty((b, n), device="cuda")
+    lib = G.L()
+    t = G.create("rfft", n)
+    ti = G.create("rfft", n, dir="inverse")
+    try:
+        torch.cuda.synchronize()
+        bi = fa.buffer_real(x.data_ptr(), n)
+        bo = fa.buffer_hermitian(re.data_ptr(), im.data_ptr(), nb)
+        fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0), "rfft")
+        by = fa.buffer_real(y.data_ptr(), n)
+        fa.check(lib.faf_execute_batch(ti, C.byref(by), C.byref(bo), b, 0, 0), "irfft")
+        torch.cuda.synchronize()
+        err = (torch.linalg.vector_norm(y - x) / torch.linalg.vector_norm(x)).item()
+        assert err < 2 * tol(n), err
+        assert float(im[:, 0].abs().max()) == 0.0 and float(im[:, -1].abs().max()) == 0.0
+        rows = [0, b // 2, b - 1]
+        xs = x[rows].cpu().numpy()
+        Xs = re[rows].cpu().numpy() + 1j * im[rows].cpu().numpy()
+        truth = np.stack([O.fft64(r.astype(np.float64))[0][:nb] for r in xs])
+        assert rel_l2(Xs, truth) < tol(n)
+    finally:
+        G.destroy(t)
+        G.destroy(ti)
+        del x, re, im, y
+        torch.cuda.empty_cache()
+
+
+def test_batch_member_equals_alone():
+    """A signal gives the same bits alone and inside a batch."""
+    say("case: batch membership")
+    n = 4096
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((37, n)).astype(np.float32)
+    t = G.create("rfft", n)
+    try:
+        allb = G.r2c(t, x)
+        for i in (0, 17, 36):
+            alone = G.r2c(t, x[i:i + 1])
+            assert np.array_equal(alone[0], allb[i])
+    finally:
+        G.destroy(t)
+
+
+# ------------------------------------------------------------------ pipeline
+
+def test_notch_pipeline_golden(golden):
+    """C2 notch: rfft -> real mask zeroing bins [ceil(N/10), N/8) -> irfft, one kernel."""
+    say("case: notch pipeline")
+    lib = G.L()
+    n = 4096
+    t = lib.chirp_compile(f"(pipeline (rfft :size {n}) (mul-spectrum) (irfft))".encode())
+    assert t, lib.faf_get_error()
+    try:
+        m = golden["notch_4096_mask"]
+        hr = G.dev(m)
+        hi = G.dev(np.zeros_like(m))
+        assert lib.chirp_bind(t, b"H", hr.data_ptr(), hi.data_ptr(), m.size) == 0
+        x = golden["notch_4096_in"]
+        y = G.real_to_real(t, x)
+        assert rel_l2(y, golden["notch_4096_out"]) < tol(n)
+        Y = np.fft.rfft(y.astype(np.float64), axis=1)
+        assert np.abs(Y[:, -(-n // 10): n // 8]).max() < 1e-3 * np.abs(Y).max()
+    finally:
+        lib.faf_destroy_transform(t)
+
+
+@pytest.mark.parametrize("n", [16, 64, 1000, 4096, 16384])
+def test_pipeline_ops_match_oracle(n):
+    say(f"case: pipeline ops n={n}")
+    lib = G.L()
+    nb = n // 2 + 1
+    rng = np.random.default_rng(n)
+    h = (rng.uniform(-1, 1, nb) + 1j * rng.uniform(-1, 1, nb)).astype(np.complex64)
+    ops = (C.c_int * 3)(2, 1, 3)
+    a1 = (C.c_uint32 * 3)(0, 0, 2)
+    a2 = (C.c_uint32 * 3)(0, 0, nb - 3)
+    t = lib.faf_create_pipeline(C.byref(fa.config_init(n)), 3, ops, a1, a2)
+    assert t, lib.faf_get_error()
+    try:
+        hr = np.ascontiguousarray(h.real)
+        hi = np.ascontiguousarray(h.imag)
+        assert lib.chirp_bind(t, b"H", hr.ctypes.data, hi.ctypes.data, nb) == 0  # host spectrum
+        x = rng.standard_normal((4, n)).astype(np.float32)
+        y = G.real_to_real(t, x)
+        ref = O.pipeline32(x, n, [2, 1, 3], [0, 0, 2], [0, 0, nb - 3], h)
+        assert rel_l2(y, ref) < 2 * tol(n)
+    finally:
+        lib.faf_destroy_transform(t)
+
+
+def test_chirp_forms():
+    say("case: chirp forms")
+    lib = G.L()
+    n = 16
+    t = lib.chirp_compile(b"(pipeline (rfft :size 16) conj (irfft))")
+    assert t, lib.faf_get_error()
+    try:
+        x = (np.arange(n, dtype=np.float32) - 8)[None]
+        y = G.real_to_real(t, x)[0]
+        assert np.allclose(y, np.r_[x[0, 0], x[0, :0:-1]], atol=1e-4)   # conj = time reversal
+    finally:
+        lib.faf_destroy_transform(t)
+    t = lib.chirp_compile(b"(pipeline (rfft :size 32) (bandpass :lo 4 :hi 4) (irfft))")
+    assert t
+    try:
+        i = np.arange(32)
+        x = (np.cos(2 * np.pi * 4 * i / 32) + np.cos(2 * np.pi * 7 * i / 32)).astype(np.float32)
+        y = G.real_to_real(t, x[None])[0]
+        assert np.allclose(y, np.cos(2 * np.pi * 4 * i / 32), atol=1e-5)
+    finally:
+        lib.faf_destroy_transform(t)
+    assert not lib.chirp_compile(b"(pipeline (rfft :size 64) (spectral notch) (irfft))")
+
+
+# ------------------------------------------------------------------ ragged (C3)
+
+C3_LENGTHS = [60, 360, 1000, 3000, 4800, 11, 17, 31, 97, 8192, 1009]
+
+
+def test_c3_ragged_persistent_launch():
+    """Config C3 shape: mixed lengths (incl. Rader primes) in one persistent launch."""
+    say("case: C3 ragged 2000 signals")
+    lib = G.L()
+    rng = np.random.default_rng(0)
+    lens = rng.choice(C3_LENGTHS, size=2000)
+    arr = (C.c_size_t * len(lens))(*[int(v) for v in lens])
+    r = lib.faf_ragged_create(arr, len(lens), 0)
+    assert r, lib.faf_get_error()
+    try:
+        tot = lib.faf_ragged_total(r)
+        assert tot == int(lens.sum())
+        xr = rng.uniform(-1, 1, tot).astype(np.float32)
+        xi = rng.uniform(-1, 1, tot).astype(np.float32)
+        dxr, dxi = G.dev(xr), G.dev(xi)
+        yr, yi = G.empty(tot), G.empty(tot)
+        torch.cuda.synchronize()
+        fa.check(lib.faf_ragged_execute(r, yr.data_ptr(), yi.data_ptr(), dxr.data_ptr(),
+                                        dxi.data_ptr()), "ragged")
+        y = G.host(yr) + 1j * G.host(yi)
+        off = np.concatenate([[0], np.cumsum(lens)])
+        x = xr + 1j * xi
+        worst = 0.0
+        for i in range(0, len(lens), 7):
+            s = slice(off[i], off[i + 1])
+            e = rel_l2(y[s], O.fft64(x[s])[0]) / tol(int(lens[i]))
+            worst = max(worst, e)
+        assert worst < 1.0, worst
+        # each signal equals the same length run alone (batched plan)
+        for n in (11, 1009, 4800):
+            i = int(np.nonzero(lens == n)[0][0])
+            s = slice(off[i], off[i + 1])
+            t = G.create("fft", n)
+            try:
+                alone = G.c2c(t, x[s][None])[0]
+            finally:
+                G.destroy(t)
+            assert rel_l2(y[s], alone) < tol(n)
+    finally:
+        lib.faf_ragged_destroy(r)
+
+
+# ------------------------------------------------------------------ STFT / CWT (C4)
+
+def test_framed_stft():
+    say("case: framed stft")
+    lib = G.L()
+    win, hop, length, recs = 1024, 256, 20000, 3
+    t = G.create("stft", win, layout="real", win_length=win, hop_length=hop)
+    try:
+        rng = np.random.default_rng(4)
+        x = rng.standard_normal((recs, length)).astype(np.float32)
+        nf = lib.faf_stft_frames(t, length)
+        assert nf == (length - win) // hop + 1
+        nb = win // 2 + 1
+        dx = G.dev(x)
+        re, im = G.empty((recs, nf, nb)), G.empty((recs, nf, nb))
+        torch.cuda.synchronize()
+        fa.check(lib.faf_execute_stft(t, re.data_ptr(), im.data_ptr(), dx.data_ptr(), length,
+                                      recs, 0), "stft")
+        S = G.host(re) + 1j * G.host(im)
+        for r in range(recs):
+            assert rel_l2(S[r], O.stft64(x[r], win, hop)) < tol(win)
+    finally:
+        G.destroy(t)
+
+
+def _cwt_create(n, fs, voices=None, fmin=None, fmax=None):
+    lib = G.L()
+    cc = lib.faf_cwt_config_init(n)
+    cc.fs = fs
+    if voices:
+        cc.voices = voices
+    if fmin:
+        cc.fmin = fmin
+    if fmax:
+        cc.fmax = fmax
+    t = lib.faf_create_cwt(C.byref(cc))
+    assert t, lib.faf_get_error()
+    return t
+
+
+def _table(t, lib):
+    rows = lib.faf_cwt_n_rows(t)
+    nb = lib.faf_cwt_n_bins(t)
+    stride = (nb + 15) & ~15
+    tab = np.zeros((rows, stride), np.float32)
+    tab[0, :nb] = np.ctypeslib.as_array(lib.faf_cwt_phi_f32(t), (nb,))
+    for j in range(rows - 1):
+        tab[j + 1, :nb] = np.ctypeslib.as_array(lib.faf_cwt_psi_f32(t, j), (nb,))
+    return tab, stride
+
+
+def test_cwt_bank_matches_reference(golden):
+    """C4 bank (n=4096, fs=16 kHz, 11 voices, fmin 62.5 Hz): 64 scales + lowpass."""
+    say("case: cwt bank tables")
+    lib = G.L()
+    f0 = 62.5
+    t = _cwt_create(4096, 16000.0, 11, f0, f0 * 2 ** (63.5 / 11))
+    try:
+        assert lib.faf_cwt_n_scales(t) == 64 and lib.faf_cwt_n_rows(t) == 65
+        tab, _ = _table(t, lib)
+        ref = golden["cwt4096_psi"]
+        assert rel_l2(tab[:, :ref.shape[1]], ref) < 1e-6
+        fr = np.zeros(64)
+        lib.faf_cwt_freqs(t, fr.ctypes.data_as(C.POINTER(C.c_double)), 64)
+        assert np.allclose(fr, golden["cwt4096_freqs"], rtol=1e-12)
+        rep = lib.faf_cwt_bank_report(t).contents
+        g = golden["cwt4096_report"]
+        assert rep.n_scales == g[0] and rep.passed == g[1] and rep.k_lo == g[7] and rep.k_hi == g[8]
+        assert abs(rep.max_abs_dev - g[2]) < 1e-9 and abs(rep.frame_cond - g[6]) < 1e-9
+        assert abs(rep.admissibility_C - g[11]) < 1e-9 * abs(g[11])
+        assert rep.frame_cond < 1.2 and rep.max_wrap_energy <= 1e-3
+    finally:
+        lib.faf_destroy_transform(t)
+
+
+def test_cwt_complex_rows_and_reference_real_rows(golden):
+    say("case: cwt rows")
+    lib = G.L()
+    t = _cwt_create(512, 1.0)
+    try:
+        tab, stride = _table(t, lib)
+        rows = tab.shape[0]
+        assert rel_l2(tab[:, :257], golden["cwt512_psi"]) < 1e-6
+        x = golden["cwt512_in"]
+        # reference's REAL output through faf_execute
+        dx = G.dev(x)
+        w = G.empty((rows, 512))
+        torch.cuda.synchronize()
+        bi = fa.buffer_real(dx.data_ptr(), 512)
+        bo = fa.buffer_real(w.data_ptr(), rows * 512)
+        fa.check(lib.faf_execute(t, C.byref(bo), C.byref(bi)), "cwt")
+        assert rel_l2(G.host(w), golden["cwt512_rows"]) < tol(512)
+        # analytic complex64 rows for a batch
+        rng = np.random.default_rng(2)
+        xs = rng.standard_normal((5, 512)).astype(np.float32)
+        dxs = G.dev(xs)
+        z = G.empty((5, rows, 512, 2))
+        torch.cuda.synchronize()
+        fa.check(lib.faf_execute_cwt_complex(t, z.data_ptr(), dxs.data_ptr(), 5, 0), "cwt complex")
+        zc = G.host(z)
+        zc = zc[..., 0] + 1j * zc[..., 1]
+        for s in range(5):
+            assert rel_l2(zc[s], O.cwt_complex64(xs[s], tab, stride)) < tol(512)
+    finally:
+        lib.faf_destroy_transform(t)
+
+
+def test_cwt_c4_segments_and_dual_inverse():
+    """C4 CWT bank on 4096-sample segments; dual-frame inverse reconstructs."""
+    say("case: cwt C4 segments + icwt")
+    lib = G.L()
+    f0 = 62.5
+    t = _cwt_create(4096, 16000.0, 11, f0, f0 * 2 ** (63.5 / 11))
+    ti = lib.faf_create_inverse(t)
+    assert ti, lib.faf_get_error()
+    try:
+        tab, stride = _table(t, lib)
+        rows = 65
+        rng = np.random.default_rng(8)
+        xs = rng.standard_normal((3, 4096)).astype(np.float32)
+        dxs = G.dev(xs)
+        z = G.empty((3, rows, 4096, 2))
+        torch.cuda.synchronize()
+        fa.check(lib.faf_execute_cwt_complex(t, z.data_ptr(), dxs.data_ptr(), 3, 0), "cwt")
+        zc = G.host(z)
+        zc = zc[..., 0] + 1j * zc[..., 1]
+        assert rel_l2(zc[1], O.cwt_complex64(xs[1], tab, stride)) < tol(4096)
+        # real rows -> dual inverse -> input (reference tests/test_cwt.cpp:348-399: <= 1e-4)
+        w = G.dev(np.ascontiguousarray(zc[0].real.astype(np.float32)))
+        y = G.empty(4096)
+        torch.cuda.synchronize()
+        bi = fa.buffer_real(w.data_ptr(), rows * 4096)
+        bo = fa.buffer_real(y.data_ptr(), 4096)
+        fa.check(lib.faf_execute(ti, C.byref(bo), C.byref(bi)), "icwt")
+        assert rel_l2(G.host(y), xs[0]) < 1e-4
+    finally:
+        lib.faf_destroy_transform(t)
+        lib.faf_destroy_transform(ti)
+
+
+def test_autotune_keeps_results():
+    say("case: autotune")
+    lib = G.L()
+    n = 4096
+    t = G.create("rfft", n)
+    try:
+        rng = np.random.default_rng(1)
+        x = rng.standard_normal((64, n)).astype(np.float32)
+        before = G.r2c(t, x)
+        spb = lib.faf_amd_autotune(t, 4096)
+        assert spb >= 1, lib.faf_get_error()
+        after = G.r2c(t, x)
+        assert rel_l2(after, before) < tol(n)
+    finally:
+        G.destroy(t)

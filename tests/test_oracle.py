@@ -1,0 +1,234 @@
+"""CPU tests: pin the oracle before it is trusted as the checker.
+
+Pins used (each chosen so a plausible slip -- wrong sign, dropped term,
+transposed index, wrong norm -- fails at least one):
+  * the reference's own known-answer tests (size policy, factor lists,
+    Hermitian multiply KATs, impulse/constant/tone answers);
+  * numpy's FFT as an independent library routine for the FP64 ground truth;
+  * the reference library compiled from its sources: golden outputs in
+    tests/golden/golden.npz, and live calls when oracle/_ref is present;
+  * closed forms and invariants (Parseval, linearity, shift, Hermitian
+    symmetry, conj = time reversal, Re(analytic CWT row) = real row).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import rel_l2, tol
+
+rng = np.random.default_rng(1234)
+
+
+def cplx(n, k=1):
+    return (rng.uniform(-1, 1, (k, n)) + 1j * rng.uniform(-1, 1, (k, n)))
+
+
+# ---------------------------------------------------------------- size policy
+
+def test_size_policy_known_answers():
+    L = O.lib()
+    # reference tests/test_mixed_radix.cpp:49-64, tests/test_sizes.cpp:56-115,
+    # tests/test_rader.cpp:74-85, tests/test_core.cpp:106-116
+    assert L.orc_is_5_smooth(1) and L.orc_is_5_smooth(12) and L.orc_is_5_smooth(4000)
+    assert not L.orc_is_5_smooth(7) and not L.orc_is_5_smooth(14)
+    assert L.orc_next_5_smooth(7) == 8 and L.orc_next_5_smooth(11) == 12
+    assert L.orc_next_5_smooth(23) == 24
+    fac = np.zeros(16, np.int32)
+    nf = C.c_int()
+    assert L.orc_factor_7smooth(12, fac, C.byref(nf)) == 0 and list(fac[:nf.value]) == [4, 3]
+    assert L.orc_factor_7smooth(28, fac, C.byref(nf)) == 0
+    assert fac[0] == 4 and fac[nf.value - 1] == 7
+    assert L.orc_factor_7smooth(4800, fac, C.byref(nf)) == 0
+    assert list(fac[:nf.value]) == [4, 4, 4, 3, 5, 5]
+    for p in (11, 13, 17, 19, 29, 31, 37, 1009):
+        assert L.orc_rader_eligible(p), p
+    for p in (7, 23, 12):
+        assert not L.orc_rader_eligible(p), p
+    assert L.orc_is_codelet_size(7) and not L.orc_is_codelet_size(11)
+    assert L.orc_fft_size_legal(14, 0) == 0 and L.orc_fft_size_legal(23, 0) == 0
+    assert L.orc_fft_size_legal(4097, 0) == 0 and L.orc_fft_size_legal(4097, 1) == 3
+    assert L.orc_fft_size_legal(1009, 0) == 2          # 1009 goes to Rader, not Bluestein
+    # primitive roots checked by brute force on the group order
+    for p in (11, 17, 31, 97, 1009):
+        g = L.orc_primitive_root(p)
+        seen = {pow(int(g), e, p) for e in range(p - 1)}
+        assert len(seen) == p - 1
+
+
+# ---------------------------------------------------------------- FP64 truth
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 8, 11, 12, 60, 97, 360, 1000, 1009, 4096])
+def test_dft64_matches_numpy_and_closed_forms(n):
+    x = cplx(n)[0]
+    assert rel_l2(O.dft64(x), np.fft.fft(x)) < 1e-13
+    assert rel_l2(O.dft64(x, inverse=True), np.fft.ifft(x) * n) < 1e-13
+    imp = np.zeros(n, complex)
+    imp[0] = 1
+    assert np.allclose(O.dft64(imp), 1.0, atol=1e-12)               # impulse -> ones
+    assert np.allclose(O.dft64(np.ones(n))[0], n)                  # constant -> n at DC
+    k = n // 3
+    tone = np.exp(2j * np.pi * k * np.arange(n) / n)
+    y = O.dft64(tone)
+    assert abs(y[k] - n) < 1e-9 * n and np.linalg.norm(np.delete(y, k)) < 1e-8 * n
+
+
+@pytest.mark.parametrize("n", [16, 1009, 4096, 4800, 16384, 65536])
+def test_fft64_matches_numpy(n):
+    x = cplx(n)[0]
+    assert rel_l2(O.fft64(x)[0], np.fft.fft(x)) < 1e-13
+    # Parseval and the shift theorem pin the sign convention independently
+    X = O.fft64(x)[0]
+    assert abs(np.sum(abs(X) ** 2) / n - np.sum(abs(x) ** 2)) < 1e-9 * np.sum(abs(x) ** 2)
+    Xs = O.fft64(np.roll(x, 1))[0]
+    assert rel_l2(Xs, X * np.exp(-2j * np.pi * np.arange(n) / n)) < 1e-12
+
+
+# ------------------------------------------- FP32 restatement of the reference
+
+C2C = [2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 16, 32, 60, 64, 360, 1000, 1024, 3000, 4096, 4800,
+       8192, 11, 17, 31, 97, 1009]
+
+
+@pytest.mark.parametrize("n", C2C)
+def test_fft32_restatement_accuracy(n):
+    x = cplx(n, 2).astype(np.complex64)
+    truth = np.fft.fft(x.astype(np.complex128), axis=1)
+    y = O.fft32(x)
+    assert rel_l2(y, truth) < tol(n)
+    yi = O.fft32(x, inverse=True)
+    assert rel_l2(yi, np.fft.ifft(x.astype(np.complex128), axis=1)) < tol(n)
+    yo = O.fft32(x, norm="ortho")
+    assert rel_l2(yo, truth / np.sqrt(n)) < tol(n)
+    yf = O.fft32(x, norm="forward")
+    assert rel_l2(yf, truth / n) < tol(n)
+
+
+@pytest.mark.parametrize("n", [23, 47, 307, 4097])
+def test_bluestein_restatement(n):
+    x = cplx(n).astype(np.complex64)
+    with pytest.raises(ValueError):
+        O.fft32(x)
+    y = O.fft32(x, bluestein=True)
+    assert rel_l2(y, np.fft.fft(x.astype(np.complex128))) < 2 * tol(n)
+
+
+def test_oracle_matches_reference_golden(golden):
+    """FP32 restatement vs the reference library's own FP32 output."""
+    for key in golden.files:
+        if key.startswith("c2c_") and key.endswith("_in"):
+            n = int(key.split("_")[1])
+            x = golden[key]
+            assert rel_l2(O.fft32(x), golden[f"c2c_{n}_fwd"]) < tol(n), n
+            assert rel_l2(O.fft32(x, inverse=True), golden[f"c2c_{n}_inv"]) < tol(n), n
+        if key.startswith("blue_") and key.endswith("_in"):
+            n = int(key.split("_")[1])
+            assert rel_l2(O.fft32(golden[key], bluestein=True), golden[f"blue_{n}_fwd"]) < 2 * tol(n)
+
+
+@pytest.mark.parametrize("n", [4, 8, 30, 64, 1000, 1024, 4096, 16384, 65536])
+def test_rfft_restatement(n):
+    x = rng.standard_normal((2, n)).astype(np.float32)
+    X = np.fft.rfft(x.astype(np.float64), axis=1)
+    assert rel_l2(O.rfft32(x), X) < tol(n)
+    assert rel_l2(O.rfft32(x, norm="ortho"), X / np.sqrt(n)) < tol(n)
+    assert rel_l2(O.irfft32(X.astype(np.complex64), n), x) < tol(n)
+    # Hermitian DC / Nyquist bins are real
+    Y = O.rfft32(x)
+    assert np.all(Y[:, 0].imag == 0) and np.all(Y[:, -1].imag == 0)
+
+
+def test_rfft_matches_reference_golden(golden):
+    for n in (1024, 4096, 16384):
+        x = golden[f"rfft_{n}_in"]
+        assert rel_l2(O.rfft32(x), golden[f"rfft_{n}_fwd"]) < tol(n)
+        assert rel_l2(O.rfft32(x, norm="ortho"), golden[f"rfft_{n}_ortho"]) < tol(n)
+        assert rel_l2(O.irfft32(golden[f"rfft_{n}_fwd"], n), golden[f"rfft_{n}_inv"]) < tol(n)
+
+
+def test_herm_mul_known_answers():
+    # reference tests/test_layout.cpp:126-153
+    f = lambda v: np.asarray(v, np.float32)
+    yr = np.zeros(5, np.float32)
+    yi = np.zeros(5, np.float32)
+    O.lib().orc_herm_mul_f32(yr, yi, f([2, 1, 0, 3, 4]), f([7, 1, 0, 2, 9]), f([3, 1, 0, 0, 5]),
+                             f([8, 0, 0, 1, 6]), 5, 0)
+    assert (yr[0], yi[0], yr[4], yi[4], yr[1], yi[1]) == (6, 0, 20, 0, 1, 1)
+    yr = np.zeros(3, np.float32)
+    yi = np.zeros(3, np.float32)
+    O.lib().orc_herm_mul_f32(yr, yi, f([1, 2, 3]), f([0, 4, 0]), f([5, 1, 7]), f([0, 1, 0]), 3, 1)
+    assert list(yr) == [5, 6, 21] and list(yi) == [0, 2, 0]
+
+
+def test_pipeline_semantics():
+    n = 64
+    x = rng.standard_normal((1, n)).astype(np.float32)
+    X = np.fft.rfft(x[0].astype(np.float64))
+    # conj in frequency = time reversal of a real signal (tests/test_spectral.cpp:144-161)
+    y = O.pipeline32(x, n, [1])[0]
+    assert np.allclose(y, np.r_[x[0, 0], x[0, :0:-1]], atol=1e-5)
+    # bandpass keeps bins [lo, hi] inclusive
+    y = O.pipeline32(x, n, [3], [4], [9])[0]
+    keep = np.zeros_like(X)
+    keep[4:10] = X[4:10]
+    assert rel_l2(y, np.fft.irfft(keep, n)) < tol(n)
+    # mul-spectrum by an identity kernel is the identity (tests/test_spectral.cpp:117-142)
+    h = np.ones(n // 2 + 1, np.complex64)
+    assert rel_l2(O.pipeline32(x, n, [2], h=h)[0], x[0]) < tol(n)
+
+
+def test_pipeline_matches_reference_golden(golden):
+    n = 4096
+    x = golden["notch_4096_in"]
+    y = O.pipeline32(x, n, [2], h=golden["notch_4096_mask"].astype(np.complex64))
+    assert rel_l2(y, golden["notch_4096_out"]) < tol(n)
+    # and the notch really removed the band
+    X = np.fft.rfft(golden["notch_4096_out"][0].astype(np.float64))
+    lo, hi = -(-n // 10), n // 8
+    assert np.abs(X[lo:hi]).max() < 1e-3 * np.abs(X).max()
+
+
+def test_hann_is_symmetric_hann():
+    for n in (8, 512, 1024):
+        assert np.allclose(O.hann32(n), np.hanning(n), atol=1e-6)
+
+
+def test_stft_composition():
+    x = rng.standard_normal(5000).astype(np.float32)
+    win, hop = 512, 128
+    S = O.stft64(x, win, hop)
+    nf = (5000 - win) // hop + 1
+    assert S.shape == (nf, win // 2 + 1)
+    w = np.hanning(win)
+    for f in (0, 7, nf - 1):
+        ref = np.fft.rfft(x[f * hop:f * hop + win].astype(np.float64) * w)
+        assert rel_l2(S[f], ref) < 1e-6
+    assert rel_l2(O.stft32(x, win, hop), S) < tol(win)
+    # a record shorter than the window is one zero-padded frame
+    s = O.stft64(x[:100], win, hop)
+    assert s.shape[0] == 1
+    assert rel_l2(s[0], np.fft.rfft(np.r_[x[:100], np.zeros(win - 100)] * w)) < 1e-6
+
+
+def test_cwt_oracle_against_reference_rows(golden):
+    psi = golden["cwt512_psi"]
+    rows, nb = psi.shape
+    stride = (nb + 15) & ~15
+    tab = np.zeros((rows, stride), np.float32)
+    tab[:, :nb] = psi
+    x = golden["cwt512_in"]
+    w = O.cwt_real32(x, tab, stride)
+    assert rel_l2(w, golden["cwt512_rows"]) < tol(512)
+    z = O.cwt_complex64(x, tab, stride)
+    assert rel_l2(z.real, golden["cwt512_rows"]) < tol(512)     # Re(analytic row) = real row
+    # analytic: wavelet rows have no negative-frequency content
+    Z = np.fft.fft(z[1:], axis=1)
+    assert np.abs(Z[:, 512 // 2 + 1:]).max() < 1e-9 * np.abs(Z).max()
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
+@pytest.mark.parametrize("n", [64, 360, 4800, 1009, 16384, 65536])
+def test_oracle_matches_live_reference(n):
+    x = cplx(n).astype(np.complex64)
+    assert rel_l2(O.fft32(x), O.ref_fft(x)) < tol(n)
