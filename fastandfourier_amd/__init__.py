@@ -153,6 +153,7 @@ _SIGS = {
     "faf_amd_autotune": (c_int, [PT, c_size_t]),
     "faf_amd_last_kernel_ms": (c_float, []), "faf_amd_set_timing": (c_int, [c_int]),
     "faf_amd_cwt_dual_f32": (POINTER(c_float), [PT, c_size_t]),
+    "faf_amd_set_specialised": (c_int, [c_int]),
 }
 
 EXPORTED = sorted(_SIGS)

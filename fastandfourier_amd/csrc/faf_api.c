@@ -47,6 +47,7 @@ static int g_cus = 256;
 
 static void rt_init(void) {
     g_rt_rc = fafk_init();
+    if (!g_rt_rc) g_rt_rc = fafk_ct_init();
     if (g_rt_rc) {
         snprintf(g_rt_msg, sizeof g_rt_msg, "%s", fafk_last_error());
         return;
@@ -78,6 +79,10 @@ int faf_amd_set_stream(void *s) { t_stream = s; return 0; }
 void *faf_amd_get_stream(void) { return t_stream; }
 int faf_amd_set_sync(int on) { t_async = !on; return 0; }
 int faf_amd_set_timing(int on) { t_timing = on; return 0; }
+
+static _Thread_local int t_no_ct;
+int faf_amd_set_specialised(int on) { t_no_ct = !on; return 0; }
+int fafi_ct_enabled(void) { return !t_no_ct; }
 float faf_amd_last_kernel_ms(void) { return t_last_ms; }
 
 int faf_amd_synchronize(void) {
@@ -782,6 +787,14 @@ static int run_stft(const faf_transform *t, planes_t *ps, long count, long idist
     (void)idist; (void)odist;
     fafh_plan *p = t->amd;
     size_t nb = p->win / 2 + 1;
+    if (p->ct) {
+        int rc = fafk_ct_stft(&p->a, p->post, p->win_dev, ps->dev[0], (long)t_stft.len,
+                              (long)t_stft.rec_dist, (long)p->hop, (long)t_stft.frames,
+                              ps->dev[1], ps->dev[2], (long)nb, (long)(t_stft.frames * nb),
+                              count, stream);
+        if (rc) fafi_err("%s", fafk_ct_last_error());
+        return rc;
+    }
     int rc = fafk_stft(&p->a, p->spb, p->post, p->win_dev, ps->dev[0], (long)t_stft.len,
                        (long)t_stft.rec_dist, (long)p->hop, (long)t_stft.frames, ps->dev[1],
                        ps->dev[2], (long)nb, (long)(t_stft.frames * nb), count, stream);

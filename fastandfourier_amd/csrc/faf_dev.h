@@ -77,6 +77,47 @@ typedef struct fafd_rplan {
     fafd_blue blue;
 } fafd_rplan;
 
+/* serving: kinds of work a persistent workgroup can pick up */
+enum {
+    FAFD_SK_C2C = 0,      /* Stockham C2C, interleaved complex in/out */
+    FAFD_SK_RADER = 1,
+    FAFD_SK_BLUE = 2,
+    FAFD_SK_R2C = 3,      /* real in, interleaved Hermitian out */
+    FAFD_SK_STFT = 4,     /* one windowed frame -> interleaved Hermitian */
+    FAFD_SK_CWT = 5       /* one segment: rfft then analytic rows [r0, r1) */
+};
+
+typedef struct fafd_splan {
+    int32_t kind;
+    int32_t len;         /* C2C length, real length (R2C/CWT) or window (STFT) */
+    int32_t threads;     /* threads per signal */
+    int32_t hop;         /* STFT */
+    fafd_fft a;          /* main FFT, R2C inner, Rader/Bluestein inner forward */
+    fafd_fft b;          /* Rader/Bluestein inner inverse; CWT length-n inverse */
+    fafd_rader rader;
+    fafd_blue blue;
+    const void *post;    /* Kiss super-twiddles (R2C, STFT, CWT) */
+    const float *win;    /* STFT window */
+    const float *psi;    /* CWT filter rows */
+    int64_t psi_stride;
+    int32_t rows;        /* CWT rows */
+    int32_t lds_per;     /* LDS float2 slots one signal needs */
+} fafd_splan;
+
+typedef struct fafd_sunit {
+    int32_t plan;
+    int32_t count;       /* signals in this unit (all of one plan) */
+    int32_t first;       /* index of the first signal in the signal table */
+    int32_t pad;
+} fafd_sunit;
+
+typedef struct fafd_ssig {
+    const float *in;     /* signal (or frame) start */
+    float *out;          /* output start */
+    int64_t avail;       /* STFT: samples left in the record from the frame start */
+    int32_t r0, r1;      /* CWT: row range of this unit */
+} fafd_ssig;
+
 /* one unit of ragged work: `count` signals of one plan */
 typedef struct fafd_unit {
     int32_t plan;

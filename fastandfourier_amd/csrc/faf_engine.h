@@ -463,6 +463,7 @@ FAF_DEV float2 apply_ops(const fafd_ops &ops, int k, float2 v) {
     return v;
 }
 
+#ifndef FAF_ENGINE_NO_PLAIN_KERNELS
 __global__ void k_pipe(fafd_fft Pf, fafd_fft Pi, int spb, const float2 *post,
                        const float2 *pre, fafd_ops ops, const float *x, long in_dist,
                        float *y, long out_dist, long count, float scale) {
@@ -499,6 +500,8 @@ __global__ void k_pipe(fafd_fft Pf, fafd_fft Pi, int spb, const float2 *post,
     GPairOut dst{y + sig * out_dist, scale};
     fft_run<true>(Pi, lds, lds, dst, sl.lt, sl.active);
 }
+#endif
+
 
 /* ------------------------------------------------------------------ */
 /* framed STFT: frame + symmetric Hann + rfft in one pass               */
@@ -517,6 +520,7 @@ struct GFrameIn {            /* windowed frame packed as z[i] = (w x)[2i] + i (w
     }
 };
 
+#ifndef FAF_ENGINE_NO_PLAIN_KERNELS
 __global__ void k_stft(fafd_fft P, int spb, const float2 *post, const float *win,
                        const float *x, long rec_len, long rec_dist, long hop, long frames,
                        float *out_re, float *out_im, long frame_dist, long rec_out_dist,
@@ -546,6 +550,8 @@ __global__ void k_stft(fafd_fft P, int spb, const float2 *post, const float *win
         ore[m] = xm.x; oim[m] = xm.y;
     }
 }
+#endif
+
 
 /* ------------------------------------------------------------------ */
 /* CWT bank: one rfft, then per row a one-sided multiply and a complex  */
@@ -618,6 +624,7 @@ __global__ void k_cwt(fafd_fft Ph, fafd_fft Pn, const float2 *post, const float 
 /* dual-frame inverse CWT (reference src/faf_cwt.c:934-981): per segment
  * acc = sum_r rfft(W_r) * dual_r, then y = irfft(acc), in one workgroup.
  * X holds the packed row spectrum, A the accumulator (nc+1 bins). */
+#ifndef FAF_ENGINE_NO_PLAIN_KERNELS
 __global__ void k_icwt(fafd_fft Pf, fafd_fft Pi, const float2 *post, const float2 *pre,
                        const float *dual, long dual_stride, int rows, const float *w,
                        long in_dist, float *y, long out_dist, long count, float scale) {
@@ -666,6 +673,8 @@ __global__ void k_icwt(fafd_fft Pf, fafd_fft Pi, const float2 *post, const float
     GPairOut dst{y + (active ? seg : 0) * out_dist, scale};
     fft_run<true>(Pi, X, X, dst, tid, active);
 }
+#endif
+
 
 /* ------------------------------------------------------------------ */
 /* four-step for lengths above one workgroup's LDS: n = n1 * n2         */
@@ -858,8 +867,8 @@ FAF_DEV void rader_body(const fafd_fft &Pf, const fafd_fft &Pi, const fafd_rader
         for (int i = lt; i < p; i += T) dst.st(i, xa.ld(i));
 }
 
-struct ChirpIn {             /* a[k] = x[k] chirp[k] for k < n, 0 up to m */
-    GSplitIn x;
+template <class S> struct ChirpIn {   /* a[k] = x[k] chirp[k] for k < n, 0 up to m */
+    S x;
     const float2 *chirp;
     int n;
     static constexpr bool is_lds = false;
@@ -878,10 +887,10 @@ template <class D> struct ChirpOut {
     }
 };
 
-template <class D>
+template <class S, class D>
 FAF_DEV void blue_body(const fafd_fft &Pf, const fafd_fft &Pi, const fafd_blue &Bq,
-                       const GSplitIn &src, const D &dst, Lds aa, int lt, int active) {
-    ChirpIn ci{src, (const float2 *)Bq.chirp, Bq.n};
+                       const S &src, const D &dst, Lds aa, int lt, int active) {
+    ChirpIn<S> ci{src, (const float2 *)Bq.chirp, Bq.n};
     fft_run<false>(Pf, ci, aa, aa, lt, active);
     __syncthreads();
     MulLds mb{aa, (const float2 *)Bq.b};
@@ -889,6 +898,7 @@ FAF_DEV void blue_body(const fafd_fft &Pf, const fafd_fft &Pi, const fafd_blue &
     fft_run<true>(Pi, mb, aa, co, lt, active);
 }
 
+#ifndef FAF_ENGINE_NO_PLAIN_KERNELS
 __global__ void k_rader(fafd_fft Pf, fafd_fft Pi, fafd_rader R, int spb, const float *in_re,
                         const float *in_im, long in_dist, float *out_re, float *out_im,
                         long out_dist, long count, float scale) {
@@ -901,7 +911,10 @@ __global__ void k_rader(fafd_fft Pf, fafd_fft Pi, fafd_rader R, int spb, const f
     GSplitOut dst{out_re + sig * out_dist, out_im + sig * out_dist, scale};
     rader_body(Pf, Pi, R, src, dst, xa, aa, sl.lt, Pf.threads, sl.active);
 }
+#endif
 
+
+#ifndef FAF_ENGINE_NO_PLAIN_KERNELS
 __global__ void k_blue(fafd_fft Pf, fafd_fft Pi, fafd_blue Bq, int spb, const float *in_re,
                        const float *in_im, long in_dist, float *out_re, float *out_im,
                        long out_dist, long count, float scale) {
@@ -912,12 +925,15 @@ __global__ void k_blue(fafd_fft Pf, fafd_fft Pi, fafd_blue Bq, int spb, const fl
     GSplitOut dst{out_re + sig * out_dist, out_im + sig * out_dist, scale};
     blue_body(Pf, Pi, Bq, src, dst, aa, sl.lt, sl.active);
 }
+#endif
+
 
 /* ------------------------------------------------------------------ */
 /* persistent ragged launch: units of same-plan signals pulled from an  */
 /* atomic queue; signals live at offsets[] in concatenated split planes */
 /* ------------------------------------------------------------------ */
 
+#ifndef FAF_ENGINE_NO_PLAIN_KERNELS
 __global__ void k_ragged(const fafd_rplan *plans, const fafd_unit *units, int nunits,
                          const int32_t *order, const long *offsets, const float *in_re,
                          const float *in_im, float *out_re, float *out_im,
@@ -954,8 +970,11 @@ __global__ void k_ragged(const fafd_rplan *plans, const fafd_unit *units, int nu
         __syncthreads();
     }
 }
+#endif
+
 
 /* y = window * x on interleaved complex rows (single-frame STFT) */
+#ifndef FAF_ENGINE_NO_PLAIN_KERNELS
 __global__ void k_window_inter(const float2 *x, long in_dist, const float *w, long n,
                                float2 *y, long count) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -963,10 +982,135 @@ __global__ void k_window_inter(const float2 *x, long in_dist, const float *w, lo
     const long b = i / n, k = i - b * n;
     y[i] = cscale(x[b * in_dist + k], w[k]);
 }
+#endif
 
+
+#ifndef FAF_ENGINE_NO_PLAIN_KERNELS
 __global__ void k_fill_u32(unsigned *p, int n, unsigned v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+#endif
+
+
+
+/* ------------------------------------------------------------------ */
+/* serving: one persistent launch per scheduling window; each unit is    */
+/* up to `count` same-plan signals with their own in/out pointers        */
+/* ------------------------------------------------------------------ */
+
+/* Kiss post-pass from LDS Z into an interleaved Hermitian row */
+FAF_DEV void post_pass_inter(const Lds &z, const float2 *post, int nc, int lt, int T,
+                             float2 *out) {
+    for (int k = lt; k <= nc / 2; k += T) {
+        const int m = nc - k;
+        float2 xk, xm;
+        if (k == 0) {
+            float2 v = z.ld(0);
+            xk = make_float2(v.x + v.y, 0.0f);
+            xm = make_float2(v.x - v.y, 0.0f);
+        } else {
+            kiss_post(z.ld(k), z.ld(m), post[k], xk, xm);
+        }
+        out[k] = xk;
+        out[m] = xm;
+    }
+}
+
+#ifndef FAF_ENGINE_NO_PLAIN_KERNELS
+__global__ void k_serve(const fafd_splan *plans, const fafd_sunit *units, int nunits,
+                        const fafd_ssig *sigs, unsigned *counter) {
+    __shared__ int s_unit;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
+    for (;;) {
+        if (tid == 0) s_unit = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int u = s_unit;
+        __syncthreads();
+        if (u >= nunits) break;
+        const fafd_sunit un = units[u];
+        const fafd_splan *pl = plans + un.plan;
+        const int T = pl->threads;
+        const int sid = tid / T, lt = tid - sid * T;
+        const int active = sid < un.count && (sid + 1) * T <= bd;
+        const fafd_ssig sg = sigs[un.first + (active ? sid : 0)];
+        float2 *base = faf_smem + sid * pl->lds_per;
+        switch (pl->kind) {
+        case FAFD_SK_C2C: {
+            Lds lds{base};
+            fft_run<false>(pl->a, GInterIn{(const float2 *)sg.in}, lds,
+                           GInterOut{(float2 *)sg.out, 1.0f}, lt, active);
+            break;
+        }
+        case FAFD_SK_RADER: {
+            Lds xa{base};
+            Lds aa{base + lpad(pl->rader.p) + 1};
+            rader_body(pl->a, pl->b, pl->rader, GInterIn{(const float2 *)sg.in},
+                       GInterOut{(float2 *)sg.out, 1.0f}, xa, aa, lt, T, active);
+            break;
+        }
+        case FAFD_SK_BLUE: {
+            Lds aa{base};
+            blue_body(pl->a, pl->b, pl->blue, GInterIn{(const float2 *)sg.in},
+                      GInterOut{(float2 *)sg.out, 1.0f}, aa, lt, active);
+            break;
+        }
+        case FAFD_SK_R2C:
+        case FAFD_SK_STFT: {
+            Lds lds{base};
+            if (pl->kind == FAFD_SK_R2C)
+                fft_run<false>(pl->a, GPairIn{sg.in}, lds, lds, lt, active);
+            else
+                fft_run<false>(pl->a, GFrameIn{sg.in, pl->win, sg.avail}, lds, lds, lt, active);
+            __syncthreads();
+            if (active) post_pass_inter(lds, (const float2 *)pl->post, pl->a.n, lt, T,
+                                        (float2 *)sg.out);
+            break;
+        }
+        default: {              /* FAFD_SK_CWT: one segment per unit */
+            const int nc = pl->a.n, n = pl->b.n;
+            Lds X{faf_smem};
+            Lds W{faf_smem + pl->a.plane};
+            const int ah = tid < pl->a.threads, an = tid < pl->b.threads;
+            fft_run<false>(pl->a, GPairIn{sg.in}, X, X, ah ? tid : 0, ah);
+            __syncthreads();
+            if (ah)
+                for (int k = tid; k <= nc / 2; k += pl->a.threads) {
+                    const int m = nc - k;
+                    float2 xk, xm;
+                    if (k == 0) {
+                        float2 z = X.ld(0);
+                        xk = make_float2(z.x + z.y, 0.0f);
+                        xm = make_float2(z.x - z.y, 0.0f);
+                    } else {
+                        kiss_post(X.ld(k), X.ld(m), ((const float2 *)pl->post)[k], xk, xm);
+                    }
+                    X.st(k, xk);
+                    X.st(m, xm);
+                }
+            __syncthreads();
+            const float sc = 1.0f / (float)n;
+            for (int r = sg.r0; r < sg.r1; ++r) {
+                OneSidedIn os{X, pl->psi + (long)r * pl->psi_stride, nc};
+                fft_run<true>(pl->b, os, W, GInterOut{(float2 *)sg.out + (long)r * n, sc},
+                              an ? tid : 0, an);
+                __syncthreads();
+            }
+            break;
+        }
+        }
+        __syncthreads();
+    }
+}
+
+/* dst[i][0..len) = src[i][0..len) for pointer tables (gather / scatter) */
+__global__ void k_copy_rows(const float *const *src, float *const *dst, long len, int rows) {
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const float *a = src[r];
+        float *b = dst[r];
+        for (long i = threadIdx.x; i < len; i += blockDim.x) b[i] = a[i];
+    }
+}
+#endif
 
 #endif

@@ -25,6 +25,7 @@ int fafi_is_codelet_size(size_t n);
 int fafi_runtime_ready(void);         /* 0 ok, -1 with error set */
 void *fafi_stream(void);              /* current thread's stream (NULL = default) */
 int fafi_sync_mode(void);
+int fafi_ct_enabled(void);            /* plans may pick compile-time kernels */
 size_t fafi_cus(void);
 
 /* ---- plan building (faf_plan.c) ---- */
@@ -60,6 +61,7 @@ typedef struct fafh_plan {
     void *rader_mem[3];
     void *blue_mem[2];
     int spb;
+    int ct;                   /* use the compile-time specialised kernel */
     float scale;
     int interleaved;          /* buffers are interleaved complex */
     void *ws;                 /* device workspace */

@@ -79,6 +79,34 @@ int fafk_ragged(const void *plans, const void *units, int nunits, const int32_t 
                 float *out_im, unsigned *counter, int grid, int block, size_t lds,
                 void *stream);
 
+int fafk_serve(const void *plans, const void *units, int nunits, const void *sigs,
+               unsigned *counter, int grid, int block, size_t lds, void *stream);
+int fafk_copy_rows(const void *src_table, const void *dst_table, long len, int rows,
+                   void *stream);
+void *fafk_host_alloc(size_t bytes);     /* pinned */
+void fafk_host_free(void *p);
+
+/* compile-time specialised kernels (faf_kernels_ct.hip): power-of-two
+ * inner lengths 256 ... 16384 */
+int fafk_ct_init(void);
+int fafk_ct_supports(int kind, int n);
+const char *fafk_ct_last_error(void);
+int fafk_ct_c2c(const fafd_fft *P, int inv, int inter, const float *in_re, const float *in_im,
+                long in_dist, float *out_re, float *out_im, long out_dist, long count,
+                float scale, void *stream);
+int fafk_ct_r2c(const fafd_fft *P, int inter_out, const void *post, const float *x,
+                long in_dist, float *out_re, float *out_im, long out_dist, long count,
+                float scale, void *stream);
+int fafk_ct_c2r(const fafd_fft *P, int inter_in, const void *pre, const float *in_re,
+                const float *in_im, long in_dist, float *y, long out_dist, long count,
+                float scale, void *stream);
+int fafk_ct_pipe(const fafd_fft *Pf, const fafd_fft *Pi, const void *post, const void *pre,
+                 const fafd_ops *ops, const float *x, long in_dist, float *y, long out_dist,
+                 long count, float scale, void *stream);
+int fafk_ct_stft(const fafd_fft *P, const void *post, const float *win, const float *x,
+                 long rec_len, long rec_dist, long hop, long frames, float *out_re,
+                 float *out_im, long frame_dist, long rec_out_dist, long count, void *stream);
+
 /* four-step I/O kinds (match the engine's enums) */
 enum { FAFK_FS_IN_SPLIT = 0, FAFK_FS_IN_PAIR = 1, FAFK_FS_IN_INTER = 2 };
 enum { FAFK_FS_OUT_SPLIT = 0, FAFK_FS_OUT_PAIR = 1, FAFK_FS_OUT_INTER = 2 };

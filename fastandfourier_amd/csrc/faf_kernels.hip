@@ -60,6 +60,7 @@ extern "C" int fafk_init(void) {
     lds_attr(k_fs_row<false, FS_OUT_PAIR>);  lds_attr(k_fs_row<true, FS_OUT_PAIR>);
     lds_attr(k_fs_row<false, FS_OUT_INTER>); lds_attr(k_fs_row<true, FS_OUT_INTER>);
     lds_attr(k_rader); lds_attr(k_blue); lds_attr(k_ragged); lds_attr(k_icwt);
+    lds_attr(k_serve);
     return 0;
 }
 
@@ -70,6 +71,17 @@ extern "C" void *fafk_malloc(size_t bytes) {
     hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
     if (e != hipSuccess) { fail("hipMalloc", e); return nullptr; }
     return p;
+}
+
+extern "C" void *fafk_host_alloc(size_t bytes) {
+    void *p = nullptr;
+    hipError_t e = hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess) { fail("hipHostMalloc", e); return nullptr; }
+    return p;
+}
+
+extern "C" void fafk_host_free(void *p) {
+    if (p) (void)hipHostFree(p);
 }
 
 extern "C" void fafk_free(void *p) {
@@ -377,5 +389,25 @@ extern "C" int fafk_ragged(const void *plans, const void *units, int nunits,
                        (const fafd_rplan *)plans, (const fafd_unit *)units, nunits, order,
                        offsets, in_re, in_im, out_re, out_im, counter);
     LAUNCH_CHECK("k_ragged");
+    return 0;
+}
+
+extern "C" int fafk_serve(const void *plans, const void *units, int nunits, const void *sigs,
+                          unsigned *counter, int grid, int block, size_t lds, void *stream) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, S(stream), counter, 1, 0u);
+    LAUNCH_CHECK("k_fill_u32");
+    hipLaunchKernelGGL(k_serve, dim3(grid), dim3(block), lds, S(stream),
+                       (const fafd_splan *)plans, (const fafd_sunit *)units, nunits,
+                       (const fafd_ssig *)sigs, counter);
+    LAUNCH_CHECK("k_serve");
+    return 0;
+}
+
+extern "C" int fafk_copy_rows(const void *src_table, const void *dst_table, long len, int rows,
+                              void *stream) {
+    if (rows <= 0) return 0;
+    hipLaunchKernelGGL(k_copy_rows, dim3(rows < 4096 ? rows : 4096), dim3(256), 0, S(stream),
+                       (const float *const *)src_table, (float *const *)dst_table, len, rows);
+    LAUNCH_CHECK("k_copy_rows");
     return 0;
 }

@@ -1,0 +1,145 @@
+/*
+ * faf_kernels_ct.hip -- C-ABI launchers of the compile-time specialised
+ * kernels (faf_engine_ct.h) for power-of-two lengths 256 ... 16384.
+ * fafk_ct_supports() tells the host which (kind, length) pairs exist.
+ */
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include "faf_engine_ct.h"
+#include "faf_kern.h"
+
+static thread_local char g_ct_err[160];
+
+static int ct_fail(const char *what, hipError_t e) {
+    snprintf(g_ct_err, sizeof g_ct_err, "%s: %s", what, hipGetErrorString(e));
+    return -1;
+}
+
+extern "C" const char *fafk_ct_last_error(void) { return g_ct_err; }
+
+static inline hipStream_t S(void *s) { return (hipStream_t)s; }
+
+static inline bool ct_len(int n) {
+    return n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096 || n == 8192 ||
+           n == 16384;
+}
+
+extern "C" int fafk_ct_supports(int kind, int n) {
+    (void)kind;
+    return ct_len(n) ? 1 : 0;
+}
+
+template <class K> static void ct_attr(K k) {
+    hipFuncAttributes fa;
+    size_t stat = 0;
+    if (hipFuncGetAttributes(&fa, (const void *)k) == hipSuccess) stat = fa.sharedSizeBytes;
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)(FAFD_MAX_LDS_BYTES - stat));
+    (void)hipGetLastError();
+}
+
+template <int N> static void ct_attrs_n() {
+    ct_attr(kc_c2c<false, false, N>); ct_attr(kc_c2c<true, false, N>);
+    ct_attr(kc_c2c<false, true, N>); ct_attr(kc_c2c<true, true, N>);
+    ct_attr(kc_r2c<false, N>); ct_attr(kc_r2c<true, N>);
+    ct_attr(kc_c2r<false, N>); ct_attr(kc_c2r<true, N>);
+    ct_attr(kc_pipe<N>); ct_attr(kc_stft<N>);
+}
+
+extern "C" int fafk_ct_init(void) {
+    ct_attrs_n<256>(); ct_attrs_n<512>(); ct_attrs_n<1024>(); ct_attrs_n<2048>();
+    ct_attrs_n<4096>(); ct_attrs_n<8192>(); ct_attrs_n<16384>();
+    return 0;
+}
+
+template <int N> struct Shape {
+    static constexpr int T = N / 16;
+    static constexpr int SPB = ct_spb<T>();
+};
+
+#define CT_SWITCH(n, CALL)                                \
+    switch (n) {                                          \
+    case 256: CALL(256); break;                           \
+    case 512: CALL(512); break;                           \
+    case 1024: CALL(1024); break;                         \
+    case 2048: CALL(2048); break;                         \
+    case 4096: CALL(4096); break;                         \
+    case 8192: CALL(8192); break;                         \
+    case 16384: CALL(16384); break;                       \
+    default:                                              \
+        snprintf(g_ct_err, sizeof g_ct_err, "no specialised kernel for length %d", n); \
+        return -1;                                        \
+    }
+
+#define GRID(N) dim3((unsigned)((count + Shape<N>::SPB - 1) / Shape<N>::SPB))
+#define BLOCK(N) dim3(Shape<N>::T * Shape<N>::SPB)
+#define LDS(N, plane) ((size_t)Shape<N>::SPB * (size_t)(plane) * sizeof(float2))
+
+extern "C" int fafk_ct_c2c(const fafd_fft *P, int inv, int inter, const float *in_re,
+                           const float *in_im, long in_dist, float *out_re, float *out_im,
+                           long out_dist, long count, float scale, void *stream) {
+#define C2C(N)                                                                           \
+    do {                                                                                 \
+        if (inv && inter) hipLaunchKernelGGL((kc_c2c<true, true, N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, in_re, in_im, in_dist, out_re, out_im, out_dist, count, scale); \
+        else if (inv) hipLaunchKernelGGL((kc_c2c<true, false, N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, in_re, in_im, in_dist, out_re, out_im, out_dist, count, scale); \
+        else if (inter) hipLaunchKernelGGL((kc_c2c<false, true, N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, in_re, in_im, in_dist, out_re, out_im, out_dist, count, scale); \
+        else hipLaunchKernelGGL((kc_c2c<false, false, N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, in_re, in_im, in_dist, out_re, out_im, out_dist, count, scale); \
+    } while (0)
+    CT_SWITCH(P->n, C2C)
+#undef C2C
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : ct_fail("kc_c2c", e);
+}
+
+extern "C" int fafk_ct_r2c(const fafd_fft *P, int inter_out, const void *post, const float *x,
+                           long in_dist, float *out_re, float *out_im, long out_dist,
+                           long count, float scale, void *stream) {
+    const float2 *po = (const float2 *)post;
+#define R2C(N)                                                                           \
+    do {                                                                                 \
+        if (inter_out) hipLaunchKernelGGL((kc_r2c<true, N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, po, x, in_dist, out_re, out_im, out_dist, count, scale); \
+        else hipLaunchKernelGGL((kc_r2c<false, N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, po, x, in_dist, out_re, out_im, out_dist, count, scale); \
+    } while (0)
+    CT_SWITCH(P->n, R2C)
+#undef R2C
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : ct_fail("kc_r2c", e);
+}
+
+extern "C" int fafk_ct_c2r(const fafd_fft *P, int inter_in, const void *pre, const float *in_re,
+                           const float *in_im, long in_dist, float *y, long out_dist, long count,
+                           float scale, void *stream) {
+    const float2 *pr = (const float2 *)pre;
+#define C2R(N)                                                                           \
+    do {                                                                                 \
+        if (inter_in) hipLaunchKernelGGL((kc_c2r<true, N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, pr, in_re, in_im, in_dist, y, out_dist, count, scale); \
+        else hipLaunchKernelGGL((kc_c2r<false, N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, pr, in_re, in_im, in_dist, y, out_dist, count, scale); \
+    } while (0)
+    CT_SWITCH(P->n, C2R)
+#undef C2R
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : ct_fail("kc_c2r", e);
+}
+
+extern "C" int fafk_ct_pipe(const fafd_fft *Pf, const fafd_fft *Pi, const void *post,
+                            const void *pre, const fafd_ops *ops, const float *x, long in_dist,
+                            float *y, long out_dist, long count, float scale, void *stream) {
+    const float2 *po = (const float2 *)post, *pr = (const float2 *)pre;
+#define PIPE(N) hipLaunchKernelGGL((kc_pipe<N>), GRID(N), BLOCK(N), LDS(N, Pf->plane), S(stream), *Pf, *Pi, po, pr, *ops, x, in_dist, y, out_dist, count, scale)
+    CT_SWITCH(Pf->n, PIPE)
+#undef PIPE
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : ct_fail("kc_pipe", e);
+}
+
+extern "C" int fafk_ct_stft(const fafd_fft *P, const void *post, const float *win,
+                            const float *x, long rec_len, long rec_dist, long hop, long frames,
+                            float *out_re, float *out_im, long frame_dist, long rec_out_dist,
+                            long count, void *stream) {
+    const float2 *po = (const float2 *)post;
+#define STFT(N) hipLaunchKernelGGL((kc_stft<N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, po, win, x, rec_len, rec_dist, hop, frames, out_re, out_im, frame_dist, rec_out_dist, count)
+    CT_SWITCH(P->n, STFT)
+#undef STFT
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : ct_fail("kc_stft", e);
+}

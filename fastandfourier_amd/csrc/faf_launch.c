@@ -17,6 +17,11 @@ static int kerr(int rc) {
     return rc;
 }
 
+static int kcerr(int rc) {
+    if (rc) fafi_err("%s", fafk_ct_last_error());
+    return rc;
+}
+
 static long chunk_signals(size_t n_complex, long count) {
     long c = (long)(CHUNK_BYTES / (n_complex * 2 * sizeof(float)));
     if (c < 1) c = 1;
@@ -30,6 +35,9 @@ int fafi_run_c2c(fafh_plan *p, const float *ir, const float *ii, long idist, flo
     const int inter = p->interleaved;
     switch (p->kind) {
     case PK_C2C:
+        if (p->ct)
+            return kcerr(fafk_ct_c2c(&p->a, p->inverse, inter, ir, ii, idist, orr, oi, odist,
+                                     count, p->scale, stream));
         return kerr(fafk_c2c(&p->a, p->inverse, inter, p->spb, ir, ii, idist, orr, oi, odist,
                              count, p->scale, stream));
     case PK_RADER:
@@ -76,6 +84,9 @@ int fafi_run_c2c(fafh_plan *p, const float *ir, const float *ii, long idist, flo
 int fafi_run_r2c(fafh_plan *p, const float *x, long idist, float *orr, float *oi, long odist,
                  long count, void *stream) {
     const int inter = p->interleaved;
+    if (p->kind == PK_R2C && p->ct)
+        return kcerr(fafk_ct_r2c(&p->a, inter, p->post, x, idist, orr, oi, odist, count,
+                                 p->scale, stream));
     if (p->kind == PK_R2C)
         return kerr(fafk_r2c(&p->a, inter, p->spb, p->post, x, idist, orr, oi, odist, count,
                              p->scale, stream));
@@ -106,6 +117,9 @@ int fafi_run_r2c(fafh_plan *p, const float *x, long idist, float *orr, float *oi
 int fafi_run_c2r(fafh_plan *p, const float *ir, const float *ii, long idist, float *y,
                  long odist, long count, void *stream) {
     const int inter = p->interleaved;
+    if (p->kind == PK_C2R && p->ct)
+        return kcerr(fafk_ct_c2r(&p->a, inter, p->pre, ir, ii, idist, y, odist, count,
+                                 p->scale, stream));
     if (p->kind == PK_C2R)
         return kerr(fafk_c2r(&p->a, inter, p->spb, p->pre, ir, ii, idist, y, odist, count,
                              p->scale, stream));
@@ -141,6 +155,9 @@ int fafi_run_pipe(fafh_plan *p, const float *x, long idist, float *y, long odist
         if (ops.code[i] == FAFD_OP_MUL && !ops.h_re) {
             ops.code[i] = FAFD_OP_NONE;   /* unbound spectrum: no-op, as the reference */
         }
+    if (p->ct)
+        return kcerr(fafk_ct_pipe(&p->a, &p->b, p->post, p->pre, &ops, x, idist, y, odist,
+                                  count, p->scale, stream));
     return kerr(fafk_pipe(&p->a, &p->b, p->spb, p->post, p->pre, &ops, x, idist, y, odist,
                           count, p->scale, stream));
 }

@@ -525,6 +525,7 @@ int fafi_plan_c2c(fafh_plan *p, size_t n, int inv, int bluestein) {
     if (fafi_radix_fits(n, 1)) {
         if (fafi_make_fft(n, inv, 1, NULL, 0, &p->a)) return -1;
         p->kind = PK_C2C;
+        p->ct = fafi_ct_enabled() && fafk_ct_supports(0, (int)n);
         int t = fafi_tuned_spb(PK_C2C, n);
         p->spb = t > 0 ? t : fafi_default_spb(&p->a, 0);
         return p->spb >= 1 ? 0 : -1;
@@ -548,6 +549,7 @@ int fafi_plan_real(fafh_plan *p, size_t n, int inv) {
     if (fafi_radix_fits(nc, 0)) {
         if (fafi_make_fft(nc, inv, 0, NULL, 0, &p->a)) return -1;
         p->kind = inv ? PK_C2R : PK_R2C;
+        p->ct = fafi_ct_enabled() && fafk_ct_supports(1, (int)nc);
         int t = fafi_tuned_spb(p->kind, n);
         p->spb = t > 0 ? t : fafi_default_spb(&p->a, 0);
         return p->spb >= 1 ? 0 : -1;
@@ -570,6 +572,7 @@ int fafi_plan_pipe(fafh_plan *p, size_t n) {
     if (fafi_make_fft(nc, 0, 0, NULL, 0, &p->a) || fafi_make_fft(nc, 1, 0, NULL, 0, &p->b))
         return -1;
     p->kind = PK_PIPE;
+    p->ct = fafi_ct_enabled() && fafk_ct_supports(3, (int)nc);
     int t = fafi_tuned_spb(PK_PIPE, n);
     p->spb = t > 0 ? t : fafi_default_spb(&p->a, 0);
     return p->spb >= 1 ? 0 : -1;
@@ -597,6 +600,7 @@ int fafi_plan_stft(fafh_plan *p, size_t win, size_t hop) {
     p->win = win;
     p->hop = hop;
     p->kind = PK_STFT;
+    p->ct = fafi_ct_enabled() && fafk_ct_supports(4, (int)nc);
     int t = fafi_tuned_spb(PK_STFT, win);
     p->spb = t > 0 ? t : fafi_default_spb(&p->a, 0);
     return p->spb >= 1 ? 0 : -1;

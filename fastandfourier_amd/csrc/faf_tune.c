@@ -43,30 +43,24 @@ static int cand_orders(const fafd_fft *P, int orders[][FAFD_MAX_STAGES], int *le
 static float time_once(fafh_plan *p, void *x, void *y, long count, void *stream) {
     void *e0 = fafk_event_create(), *e1 = fafk_event_create();
     float *xf = x, *yf = y;
-    size_t n = p->n;
+    size_t n = p->n, nb = n / 2 + 1;
     int rc = 0;
     fafk_event_record(e0, stream);
     for (int rep = 0; rep < 3 && !rc; ++rep) {
         switch (p->kind) {
         case PK_C2C:
-            rc = fafk_c2c(&p->a, p->inverse, 0, p->spb, xf, xf + count * n, (long)n, yf,
-                          yf + count * n, (long)n, count, 1.0f, stream);
+            rc = fafi_run_c2c(p, xf, xf + count * n, (long)n, yf, yf + count * n, (long)n,
+                              count, stream);
             break;
         case PK_R2C:
-            rc = fafk_r2c(&p->a, 0, p->spb, p->post, xf, (long)n, yf, yf + count * (n / 2 + 1),
-                          (long)(n / 2 + 1), count, 1.0f, stream);
+            rc = fafi_run_r2c(p, xf, (long)n, yf, yf + count * nb, (long)nb, count, stream);
             break;
         case PK_C2R:
-            rc = fafk_c2r(&p->a, 0, p->spb, p->pre, xf, xf + count * (n / 2 + 1),
-                          (long)(n / 2 + 1), yf, (long)n, count, 1.0f, stream);
+            rc = fafi_run_c2r(p, xf, xf + count * nb, (long)nb, yf, (long)n, count, stream);
             break;
-        case PK_PIPE: {
-            fafd_ops none;
-            memset(&none, 0, sizeof none);
-            rc = fafk_pipe(&p->a, &p->b, p->spb, p->post, p->pre, &none, xf, (long)n, yf,
-                           (long)n, count, 1.0f, stream);
+        case PK_PIPE:
+            rc = fafi_run_pipe(p, xf, (long)n, yf, (long)n, count, stream);
             break;
-        }
         default:
             rc = -1;
         }
@@ -104,8 +98,15 @@ int faf_amd_autotune(faf_transform *t, size_t count) {
     int inv = p->kind == PK_C2R ? 1 : p->inverse;
     size_t fft_n = (p->kind == PK_C2C) ? n : n / 2;
     fafd_fft best_a = p->a, best_b = p->b;
-    int best_spb = p->spb;
+    int best_spb = p->spb, best_ct = 0;
     float best = 1e30f;
+    if (p->ct) {               /* the specialised kernel is one more candidate */
+        fafh_plan trial = *p;
+        time_once(&trial, x, y, (long)count, stream);
+        best = time_once(&trial, x, y, (long)count, stream);
+        if (best <= 0.0f) best = 1e30f;
+        else best_ct = 1;
+    }
     int orders[MAX_CAND_ORDERS][FAFD_MAX_STAGES], lens[MAX_CAND_ORDERS];
     int no = cand_orders(&p->a, orders, lens);
     for (int o = 0; o < no; ++o) {
@@ -117,6 +118,7 @@ int faf_amd_autotune(faf_transform *t, size_t count) {
             if (spb * a.threads > 1024) break;
             if ((size_t)spb * a.plane * 8 > FAFD_MAX_LDS_BYTES) break;
             fafh_plan trial = *p;
+            trial.ct = 0;
             trial.a = a;
             trial.b = b;
             trial.spb = spb;
@@ -127,15 +129,19 @@ int faf_amd_autotune(faf_transform *t, size_t count) {
                 best_a = a;
                 best_b = b;
                 best_spb = spb;
+                best_ct = 0;
             }
         }
     }
     fafk_free(x);
     fafk_free(y);
     if (best >= 1e30f) { fafi_err("autotune: no candidate ran"); return -1; }
-    p->a = best_a;
-    p->b = best_b;
-    p->spb = best_spb;
+    if (!best_ct) {
+        p->a = best_a;
+        p->b = best_b;
+        p->spb = best_spb;
+    }
+    p->ct = best_ct;
     double gbps = (double)count * (double)n * 8.0 * 3.0 / (best * 1e-3) / 1e9;
     fafi_tune_record(p->kind, n, best_spb, gbps);
     return best_spb;

@@ -74,6 +74,12 @@ faf_transform *faf_create_pipeline(const faf_config *cfg, int nops, const int *o
  * process).  Returns the chosen signals-per-workgroup, or -1. */
 int faf_amd_autotune(faf_transform *t, size_t count);
 
+/* ---- kernel selection (thread-local, read at create time) ----
+ * Power-of-two lengths 256..16384 (C2C) and 512..32768 (R2C/C2R, fused
+ * pipeline, STFT window) use compile-time specialised kernels by default;
+ * 0 makes later creates use the runtime-generic radix kernels instead. */
+int faf_amd_set_specialised(int on);
+
 /* ---- dual-frame filter row of a CWT/ICWT bank (packed Hermitian, n/2+1
  * bins; row 0 is the lowpass when present).  Valid until destroy. ---- */
 const float *faf_amd_cwt_dual_f32(const faf_transform *t, size_t row);

@@ -557,6 +557,40 @@ def test_cwt_c4_segments_and_dual_inverse():
         lib.faf_destroy_transform(ti)
 
 
+@pytest.mark.parametrize("n", [256, 1024, 4096, 16384])
+def test_generic_and_specialised_kernels_agree(n):
+    """Power-of-two lengths run on compile-time specialised kernels by
+    default; the runtime-generic radix kernels must give the same answers."""
+    say(f"case: generic vs specialised n={n}")
+    lib = G.L()
+    rng = np.random.default_rng(n + 7)
+    x = cplx(rng, 3, n)
+    xr = rng.standard_normal((3, 2 * n)).astype(np.float32)
+    out = {}
+    for spec in (1, 0):
+        lib.faf_amd_set_specialised(spec)
+        t = G.create("fft", n)
+        ti = G.create("fft", n, dir="inverse")
+        r = G.create("rfft", 2 * n)
+        ri = G.create("rfft", 2 * n, dir="inverse")
+        try:
+            y = G.c2c(t, x)
+            X = G.r2c(r, xr)
+            out[spec] = (y, G.c2c(ti, y), X, G.c2r(ri, X, 2 * n))
+        finally:
+            for h in (t, ti, r, ri):
+                G.destroy(h)
+    lib.faf_amd_set_specialised(1)
+    truth = truth_fft(x)
+    rtruth = np.stack([O.fft64(v.astype(np.float64))[0][: n + 1] for v in xr])
+    for spec in (1, 0):
+        y, z, X, w = out[spec]
+        assert rel_l2(y, truth) < tol(n), spec
+        assert rel_l2(z, x) < 2 * tol(n), spec
+        assert rel_l2(X, rtruth) < tol(2 * n), spec
+        assert rel_l2(w, xr) < 2 * tol(2 * n), spec
+
+
 def test_autotune_keeps_results():
     say("case: autotune")
     lib = G.L()
