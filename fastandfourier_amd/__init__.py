@@ -21,6 +21,7 @@ __all__ = [
     "lib", "load", "FafConfig", "FafBuffer", "FafCwtConfig", "FafLpReport", "FafTransform",
     "config_init", "buffer_real", "buffer_split", "buffer_hermitian", "buffer_interleaved",
     "FafError", "check", "LAYOUT", "NORM", "TYPE", "PREC", "FLAG_BLUESTEIN", "EXPORTED",
+    "FafServerConfig", "FafServerStats", "REQ", "REQ_DISCARD",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -90,6 +91,28 @@ FafTransform._fields_ = [
 PT = POINTER(FafTransform)
 PB = POINTER(FafBuffer)
 
+# serving loop (include/faf_amd.h)
+REQ = {"rfft": 0, "c2c": 1, "stft": 2, "cwt": 3}
+REQ_DISCARD = 0x1
+
+
+class FafServerConfig(Structure):
+    _fields_ = [("hbm_budget", c_size_t), ("window_us", c_double), ("streams", c_int),
+                ("max_batch", c_int), ("plain", c_int), ("stft_win", c_size_t),
+                ("stft_hop", c_size_t), ("cwt_fs", c_double), ("cwt_voices", C.c_uint),
+                ("cwt_octaves", c_double)]
+
+
+class FafServerStats(Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("launches", C.c_uint64), ("mean_batch", c_double), ("p50_us", c_double),
+                ("p99_us", c_double), ("max_us", c_double), ("mean_us", c_double),
+                ("span_s", c_double), ("req_per_s", c_double), ("gb_per_s", c_double),
+                ("peak_hbm_bytes", c_size_t), ("admission_waits", C.c_uint64)]
+
+
+PS = c_void_p
+
 # every symbol the public headers declare, with its ctypes signature
 _SIGS = {
     "faf_init": (c_int, []), "faf_cleanup": (None, []), "faf_version": (c_char_p, []),
@@ -154,6 +177,22 @@ _SIGS = {
     "faf_amd_last_kernel_ms": (c_float, []), "faf_amd_set_timing": (c_int, [c_int]),
     "faf_amd_cwt_dual_f32": (POINTER(c_float), [PT, c_size_t]),
     "faf_amd_set_specialised": (c_int, [c_int]),
+    "faf_server_config_init": (FafServerConfig, []),
+    "faf_server_create": (PS, [POINTER(FafServerConfig)]),
+    "faf_server_destroy": (None, [PS]),
+    "faf_server_prepare": (c_int, [PS, c_int, c_size_t]),
+    "faf_server_set_plain": (c_int, [PS, c_int]),
+    "faf_server_result_bytes": (c_size_t, [PS, c_int, c_size_t]),
+    "faf_server_submit": (C.c_int64, [PS, c_int, c_size_t, c_void_p, c_uint32]),
+    "faf_server_poll": (c_int, [PS, C.c_int64]),
+    "faf_server_wait": (c_int, [PS, C.c_int64]),
+    "faf_server_result": (c_void_p, [PS, C.c_int64, POINTER(c_size_t)]),
+    "faf_server_release": (c_int, [PS, C.c_int64]),
+    "faf_server_drain": (c_int, [PS]),
+    "faf_server_stats_get": (c_int, [PS, POINTER(FafServerStats)]),
+    "faf_server_stats_reset": (None, [PS]),
+    "faf_server_replay": (c_int, [PS, c_void_p, c_void_p, c_void_p, c_void_p, c_size_t,
+                                  c_uint32, c_void_p]),
 }
 
 EXPORTED = sorted(_SIGS)

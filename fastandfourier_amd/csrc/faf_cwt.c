@@ -393,6 +393,25 @@ static void *upload_f32(const float *h, size_t count) {
     return d;
 }
 
+/* the rfft of a segment and one length-n inverse row side by side in LDS */
+static int cwt_fits_one_workgroup(size_t n) {
+    size_t nc = n / 2;
+    if (!fafi_radix_fits(nc, 0) || !fafi_radix_fits(n, 0)) return 0;
+    size_t planes = (nc + (nc >> 4) + 1) + (n + (n >> 4) + 1);
+    return planes * 8 <= FAFD_MAX_LDS_BYTES;
+}
+
+/* sub-plan of a long-segment CWT: interleaved device buffers, given scale */
+static fafh_plan *cwt_sub(size_t n, int real, int inv, float scale) {
+    fafh_plan *q = fafi_plan_new();
+    if (!q) { fafi_err("cwt: alloc failed"); return NULL; }
+    int rc = real ? fafi_plan_real(q, n, inv) : fafi_plan_c2c(q, n, inv, 0);
+    if (rc) { fafi_plan_free(q); return NULL; }
+    q->interleaved = 1;
+    q->scale = scale;
+    return q;
+}
+
 int fafi_cwt_build(fafh_plan *p, const faf_cwt_config *cfg, int inverse_kind) {
     fafi_cwt_bank *b = bank_build(cfg);
     if (!b) return -1;
@@ -405,20 +424,29 @@ int fafi_cwt_build(fafh_plan *p, const faf_cwt_config *cfg, int inverse_kind) {
     p->post = fafi_kiss_table(nc, 0);
     p->pre = fafi_kiss_table(nc, 1);
     if (!p->post || !p->pre) return -1;
-    if (!fafi_radix_fits(nc, 0) || fafi_make_fft(nc, 0, 0, NULL, 0, &p->a)) {
-        fafi_err("cwt: length %zu does not fit one workgroup", n);
-        return -1;
-    }
+    if (n % 2) { fafi_err("cwt: length %zu must be even", n); return -1; }
     if (inverse_kind < 0) {
-        /* forward: analytic rows need a length-n inverse FFT */
-        if (!fafi_radix_fits(n, 0) || fafi_make_fft(n, 1, 0, NULL, 0, &p->full_inv)) {
-            fafi_err("cwt: length %zu does not fit one workgroup", n);
-            return -1;
-        }
-        p->psi_dev = upload_f32(b->psi32, b->n_rows * b->stride);
         p->kind = PK_CWT;
         p->scale = (float)(1.0 / (double)n);
-        return p->psi_dev ? 0 : -1;
+        p->psi_dev = upload_f32(b->psi32, b->n_rows * b->stride);
+        if (!p->psi_dev) return -1;
+        if (cwt_fits_one_workgroup(n)) {
+            /* one kernel: rfft in LDS, then every row's length-n inverse */
+            if (fafi_make_fft(nc, 0, 0, NULL, 0, &p->a) ||
+                fafi_make_fft(n, 1, 0, NULL, 0, &p->full_inv))
+                return -1;
+            return 0;
+        }
+        /* longer segments: rfft(n), spread the filtered rows, then one
+         * batched inverse over all rows (four-step where n needs it) */
+        p->sub_r2c = cwt_sub(n, 1, 0, 1.0f);
+        p->sub_c2c = p->sub_r2c ? cwt_sub(n, 0, 1, p->scale) : NULL;
+        p->sub_c2r = p->sub_c2c ? cwt_sub(n, 1, 1, p->scale) : NULL;
+        return p->sub_c2r ? 0 : -1;
+    }
+    if (!fafi_radix_fits(nc, 0) || fafi_make_fft(nc, 0, 0, NULL, 0, &p->a)) {
+        fafi_err("cwt: inverse of length %zu does not fit one workgroup", n);
+        return -1;
     }
     if (fafi_make_fft(nc, 1, 0, NULL, 0, &p->b)) return -1;
     p->dual_dev = upload_f32(b->dual32, b->n_rows * b->stride);
@@ -436,8 +464,98 @@ void fafi_cwt_free(fafh_plan *p) {
     p->psi_dev = p->dual_dev = NULL;
 }
 
+/* long-segment CWT: segments per chunk, so the filtered rows stay <= 256 MB */
+static long cwt_chunk(const fafh_plan *p, long count) {
+    size_t per = (size_t)p->rows_total * (p->n + 2) * 8;
+    long c = (long)(((size_t)256 << 20) / per);
+    if (c < 1) c = 1;
+    return c < count ? c : count;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t fafi_cwt_ws_bytes(const fafh_plan *p, long count, int complex_out) {
+    if (!p->sub_r2c || count <= 0) return 0;
+    const size_t nc = p->n / 2, rows = (size_t)p->rows_total;
+    const long c = cwt_chunk(p, count);
+    size_t z = align256((size_t)c * (nc + 1) * 8);
+    size_t sp = complex_out ? 0 : align256((size_t)c * rows * (nc + 1) * 8);
+    size_t s1 = fafi_run_ws_bytes(p->sub_r2c, c);
+    size_t s2 = fafi_run_ws_bytes(complex_out ? p->sub_c2c : p->sub_c2r, c * (long)rows);
+    return z + sp + (s1 > s2 ? s1 : s2);
+}
+
+/* run a sub-plan on lent workspace */
+#define LENT(q, w, wb, call)              \
+    ((q)->ws_ext = (w), (q)->ws_ext_bytes = (wb), rc = (call), (q)->ws_ext = NULL, \
+     (q)->ws_ext_bytes = 0, rc)
+
+static int run_cwt_long(fafh_plan *p, const float *x, long idist, float *out, long odist,
+                        long count, int complex_out, void *stream) {
+    const long n = (long)p->n, nc = n / 2, rows = p->rows_total;
+    const long c = cwt_chunk(p, count);
+    const size_t need = fafi_cwt_ws_bytes(p, count, complex_out);
+    char *ws = fafi_workspace(p, need);
+    if (!ws) return -1;
+    float *Z = (float *)ws;
+    size_t zb = align256((size_t)c * (size_t)(nc + 1) * 8);
+    size_t sb = complex_out ? 0 : align256((size_t)c * (size_t)rows * (size_t)(nc + 1) * 8);
+    float *S = (float *)(ws + zb);
+    char *sub = ws + zb + sb;
+    const size_t subb = need - zb - sb;
+    int rc = 0;
+    for (long b0 = 0; b0 < count; b0 += c) {
+        const long cc = count - b0 < c ? count - b0 : c;
+        if (LENT(p->sub_r2c, sub, subb,
+                 fafi_run_r2c(p->sub_r2c, x + b0 * idist, idist, Z, NULL, nc + 1, cc, stream)))
+            return -1;
+        if (complex_out) {
+            /* analytic rows: doubled one-sided spectrum, inverse C2C in place */
+            float *o = out + 2 * b0 * odist;
+            if (fafk_cwt_spread(Z, nc + 1, p->psi_dev, p->psi_stride, (int)nc, 1, o, n, odist,
+                                (int)rows, cc, stream)) {
+                fafi_err("%s", fafk_last_error());
+                return -1;
+            }
+            if (odist == rows * n) {
+                if (LENT(p->sub_c2c, sub, subb,
+                         fafi_run_c2c(p->sub_c2c, o, NULL, n, o, NULL, n, cc * rows, stream)))
+                    return -1;
+            } else {
+                for (long s = 0; s < cc; ++s) {
+                    float *os = o + 2 * s * odist;
+                    if (LENT(p->sub_c2c, sub, subb,
+                             fafi_run_c2c(p->sub_c2c, os, NULL, n, os, NULL, n, rows, stream)))
+                        return -1;
+                }
+            }
+        } else {
+            /* real rows = irfft(X psi_r): spread into S, batched C2R */
+            if (fafk_cwt_spread(Z, nc + 1, p->psi_dev, p->psi_stride, (int)nc, 0, S, nc + 1,
+                                rows * (nc + 1), (int)rows, cc, stream)) {
+                fafi_err("%s", fafk_last_error());
+                return -1;
+            }
+            float *o = out + b0 * odist;
+            if (odist == rows * n) {
+                if (LENT(p->sub_c2r, sub, subb,
+                         fafi_run_c2r(p->sub_c2r, S, NULL, nc + 1, o, n, cc * rows, stream)))
+                    return -1;
+            } else {
+                for (long s = 0; s < cc; ++s)
+                    if (LENT(p->sub_c2r, sub, subb,
+                             fafi_run_c2r(p->sub_c2r, S + 2 * s * rows * (nc + 1), NULL, nc + 1,
+                                          o + s * odist, n, rows, stream)))
+                        return -1;
+            }
+        }
+    }
+    return 0;
+}
+
 int fafi_run_cwt(fafh_plan *p, const float *x, long idist, float *out, long odist,
                  long count, int complex_out, void *stream) {
+    if (p->sub_r2c) return run_cwt_long(p, x, idist, out, odist, count, complex_out, stream);
     int rows = p->rows_total;
     /* enough workgroups to fill the device: split the rows of a segment
      * across workgroups when there are few segments */
@@ -447,8 +565,7 @@ int fafi_run_cwt(fafh_plan *p, const float *x, long idist, float *out, long odis
     if (groups > rows) groups = rows;
     int rpw = (rows + groups - 1) / groups;
     int rc = fafk_cwt(&p->a, &p->full_inv, !complex_out, p->post, p->psi_dev, p->psi_stride,
-                      rows, rpw, x, idist, out, complex_out ? odist * 1 : odist, count,
-                      p->scale, stream);
+                      rows, rpw, x, idist, out, odist, count, p->scale, stream);
     if (rc) fafi_err("%s", fafk_last_error());
     return rc;
 }

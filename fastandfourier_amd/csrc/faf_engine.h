@@ -1020,10 +1020,13 @@ FAF_DEV void post_pass_inter(const Lds &z, const float2 *post, int nc, int lt, i
 #ifndef FAF_ENGINE_NO_PLAIN_KERNELS
 __global__ void k_serve(const fafd_splan *plans, const fafd_sunit *units, int nunits,
                         const fafd_ssig *sigs, unsigned *counter) {
+    /* with a counter, workgroups pull units from it until the table is
+     * drained; without one, workgroup b runs unit b only */
     __shared__ int s_unit;
     const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
-    for (;;) {
-        if (tid == 0) s_unit = (int)atomicAdd(counter, 1u);
+    for (int it = 0;; ++it) {
+        if (tid == 0)
+            s_unit = counter ? (int)atomicAdd(counter, 1u) : (it == 0 ? (int)blockIdx.x : nunits);
         __syncthreads();
         const int u = s_unit;
         __syncthreads();
@@ -1103,13 +1106,27 @@ __global__ void k_serve(const fafd_splan *plans, const fafd_sunit *units, int nu
     }
 }
 
-/* dst[i][0..len) = src[i][0..len) for pointer tables (gather / scatter) */
-__global__ void k_copy_rows(const float *const *src, float *const *dst, long len, int rows) {
-    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
-        const float *a = src[r];
-        float *b = dst[r];
-        for (long i = threadIdx.x; i < len; i += blockDim.x) b[i] = a[i];
+/* filtered one-sided CWT row spectra of long segments, for a separate
+ * inverse FFT.  X: packed spectra [count][x_dist] (nc+1 bins used).
+ * analytic: out[seg][r][k], k < 2 nc, holds X psi_r doubled on 0 < k < nc,
+ * undoubled at k = 0 and nc, zero above nc (its inverse C2C gives the
+ * analytic row).  Otherwise k <= nc holds X psi_r (for an inverse C2R).
+ * grid: x = bins / 256, y = rows, z = segments. */
+__global__ void k_cwt_spread(const float2 *X, long x_dist, const float *psi, long psi_stride,
+                             int nc, int analytic, float2 *out, long row_len, long seg_dist,
+                             int rows) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = (int)blockIdx.y;
+    const long seg = blockIdx.z;
+    if (k >= row_len || r >= rows) return;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (k <= nc) {
+        const float2 z = X[seg * x_dist + k];
+        float p = psi[(long)r * psi_stride + k];
+        if (analytic && k != 0 && k != nc) p *= 2.0f;
+        v = make_float2(z.x * p, z.y * p);
     }
+    out[seg * seg_dist + (long)r * row_len + k] = v;
 }
 #endif
 

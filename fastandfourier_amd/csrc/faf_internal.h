@@ -84,6 +84,14 @@ typedef struct fafh_plan {
     long psi_stride;
     int rows_total, rows_per_wg;
     fafd_fft full_inv;        /* length-n inverse for analytic CWT rows */
+    /* CWT segments longer than one workgroup holds: rfft, filter spread,
+     * then a batched inverse over the rows (C2C for analytic rows, C2R for
+     * real rows), each a plan of its own */
+    struct fafh_plan *sub_r2c, *sub_c2c, *sub_c2r;
+    /* caller-lent device workspace, used instead of ws while set (the
+     * serving loop lends each request its own so streams never share) */
+    void *ws_ext;
+    size_t ws_ext_bytes;
 } fafh_plan;
 
 fafh_plan *fafi_plan_new(void);
@@ -107,6 +115,7 @@ int fafi_run_c2r(fafh_plan *p, const float *ir, const float *ii, long idist, flo
 int fafi_run_pipe(fafh_plan *p, const float *x, long idist, float *y, long odist, long count,
                   void *stream);
 int fafi_pipe_refresh(fafh_plan *p, void *stream);
+size_t fafi_run_ws_bytes(const fafh_plan *p, long count);   /* workspace one run needs */
 
 /* CWT bank (faf_cwt.c) */
 int fafi_cwt_build(fafh_plan *p, const faf_cwt_config *cfg, int inverse_kind);
@@ -115,5 +124,6 @@ int fafi_run_cwt(fafh_plan *p, const float *x, long idist, float *out, long odis
                  long count, int complex_out, void *stream);
 int fafi_run_icwt(fafh_plan *p, const float *w, long idist, float *y, long odist,
                   long count, void *stream);
+size_t fafi_cwt_ws_bytes(const fafh_plan *p, long count, int complex_out);
 
 #endif

@@ -81,8 +81,9 @@ int fafk_ragged(const void *plans, const void *units, int nunits, const int32_t 
 
 int fafk_serve(const void *plans, const void *units, int nunits, const void *sigs,
                unsigned *counter, int grid, int block, size_t lds, void *stream);
-int fafk_copy_rows(const void *src_table, const void *dst_table, long len, int rows,
-                   void *stream);
+int fafk_cwt_spread(const float *X, long x_dist, const float *psi, long psi_stride, int nc,
+                    int analytic, float *out, long row_len, long seg_dist, int rows,
+                    long count, void *stream);   /* dists in complex elements */
 void *fafk_host_alloc(size_t bytes);     /* pinned */
 void fafk_host_free(void *p);
 

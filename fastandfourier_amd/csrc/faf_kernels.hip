@@ -394,8 +394,13 @@ extern "C" int fafk_ragged(const void *plans, const void *units, int nunits,
 
 extern "C" int fafk_serve(const void *plans, const void *units, int nunits, const void *sigs,
                           unsigned *counter, int grid, int block, size_t lds, void *stream) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, S(stream), counter, 1, 0u);
-    LAUNCH_CHECK("k_fill_u32");
+    if (nunits <= 0) return 0;
+    if (counter) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, S(stream), counter, 1, 0u);
+        LAUNCH_CHECK("k_fill_u32");
+    } else {
+        grid = nunits;
+    }
     hipLaunchKernelGGL(k_serve, dim3(grid), dim3(block), lds, S(stream),
                        (const fafd_splan *)plans, (const fafd_sunit *)units, nunits,
                        (const fafd_ssig *)sigs, counter);
@@ -403,11 +408,14 @@ extern "C" int fafk_serve(const void *plans, const void *units, int nunits, cons
     return 0;
 }
 
-extern "C" int fafk_copy_rows(const void *src_table, const void *dst_table, long len, int rows,
-                              void *stream) {
-    if (rows <= 0) return 0;
-    hipLaunchKernelGGL(k_copy_rows, dim3(rows < 4096 ? rows : 4096), dim3(256), 0, S(stream),
-                       (const float *const *)src_table, (float *const *)dst_table, len, rows);
-    LAUNCH_CHECK("k_copy_rows");
+extern "C" int fafk_cwt_spread(const float *X, long x_dist, const float *psi, long psi_stride,
+                               int nc, int analytic, float *out, long row_len, long seg_dist,
+                               int rows, long count, void *stream) {
+    if (count <= 0 || rows <= 0) return 0;
+    if (count > 65535 || rows > 65535) return fail("k_cwt_spread grid", hipErrorInvalidValue);
+    dim3 grid((unsigned)((row_len + 255) / 256), (unsigned)rows, (unsigned)count);
+    hipLaunchKernelGGL(k_cwt_spread, grid, dim3(256), 0, S(stream), (const float2 *)X, x_dist,
+                       psi, psi_stride, nc, analytic, (float2 *)out, row_len, seg_dist, rows);
+    LAUNCH_CHECK("k_cwt_spread");
     return 0;
 }

@@ -28,6 +28,21 @@ static long chunk_signals(size_t n_complex, long count) {
     return c < count ? c : count;
 }
 
+size_t fafi_run_ws_bytes(const fafh_plan *p, long count) {
+    if (count <= 0) return 0;
+    switch (p->kind) {
+    case PK_C2C_4S:
+        return (size_t)chunk_signals(p->n, count) * p->n * 2 * sizeof(float);
+    case PK_R2C_4S:
+    case PK_C2R_4S:
+        return (size_t)chunk_signals(p->n / 2, count) * (p->n / 2) * 4 * sizeof(float);
+    case PK_CWT:
+        return fafi_cwt_ws_bytes(p, count, 1);
+    default:
+        return 0;
+    }
+}
+
 /* ------------------------------------------------------------------ */
 
 int fafi_run_c2c(fafh_plan *p, const float *ir, const float *ii, long idist, float *orr,

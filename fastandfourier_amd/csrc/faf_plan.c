@@ -354,6 +354,9 @@ fafh_plan *fafi_plan_new(void) {
 void fafi_plan_free(fafh_plan *p) {
     if (!p) return;
     fafi_cwt_free(p);
+    fafi_plan_free(p->sub_r2c);
+    fafi_plan_free(p->sub_c2c);
+    fafi_plan_free(p->sub_c2r);
     for (int i = 0; i < 3; ++i) fafk_free(p->rader_mem[i]);
     for (int i = 0; i < 2; ++i) fafk_free(p->blue_mem[i]);
     fafk_free(p->ws);
@@ -364,6 +367,11 @@ void fafi_plan_free(fafh_plan *p) {
 }
 
 void *fafi_workspace(fafh_plan *p, size_t bytes) {
+    if (p->ws_ext) {
+        if (p->ws_ext_bytes >= bytes) return p->ws_ext;
+        fafi_err("lent workspace holds %zu bytes, the run needs %zu", p->ws_ext_bytes, bytes);
+        return NULL;
+    }
     if (p->ws_bytes >= bytes) return p->ws;
     fafk_sync(fafi_stream());
     fafk_free(p->ws);

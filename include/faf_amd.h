@@ -88,6 +88,87 @@ const float *faf_amd_cwt_dual_f32(const faf_transform *t, size_t row);
 float faf_amd_last_kernel_ms(void);
 int faf_amd_set_timing(int on);
 
+/* ---- serving loop with continuous batching ----
+ * Requests carry a device-resident input and get their result in device
+ * memory that the server allocates from an arena of cfg.hbm_budget bytes.
+ * A worker thread admits requests in arrival order while their result (and
+ * workspace) fits the budget, gathers admitted requests into one launch of
+ * the persistent serving kernel whenever a stream slot is free (at once when
+ * the device is idle, else after window_us or when max_batch is reached) and
+ * completes them as the slot's events fire, so requests that arrive while
+ * others run are admitted and overlap with them on the device.
+ *
+ * Request kinds and results (interleaved complex64, unnormalised forward):
+ *   RFFT  n even real samples        -> n/2+1 bins
+ *   C2C   n interleaved complex      -> n bins
+ *   STFT  n real samples             -> frames x (win/2+1), frames =
+ *         1 + (n - win)/hop (one zero-padded frame when n < win), symmetric
+ *         Hann window, no centre padding (as faf_execute_stft)
+ *   CWT   n real samples (n even)    -> rows x n analytic rows (as
+ *         faf_execute_cwt_complex) of a Morse bank with cwt_voices voices
+ *         per octave from fmin = 16 fs/n to fmin * 2^cwt_octaves, plus the
+ *         lowpass row. */
+typedef enum {
+    FAF_REQ_RFFT = 0, FAF_REQ_C2C = 1, FAF_REQ_STFT = 2, FAF_REQ_CWT = 3
+} faf_req_kind;
+
+#define FAF_REQ_DISCARD 0x1u   /* free the result as soon as it completes */
+
+typedef struct faf_server_config {
+    size_t hbm_budget;      /* bytes of results + workspace in flight (default 64 GiB) */
+    double window_us;       /* batching window while the device is busy (default 60) */
+    int streams;            /* stream slots, 1..8 (default 3) */
+    int max_batch;          /* requests per launch (default 1024) */
+    int plain;              /* 1: no batching, one launch + synchronize per request */
+    size_t stft_win;        /* default 512 */
+    size_t stft_hop;        /* default 128 */
+    double cwt_fs;          /* default 16000 */
+    unsigned cwt_voices;    /* default 8 */
+    double cwt_octaves;     /* default 31.5 / 8 (32 scales) */
+} faf_server_config;
+
+typedef struct faf_server faf_server;
+
+typedef struct faf_server_stats {
+    uint64_t submitted, completed, failed;
+    uint64_t launches;           /* batches dispatched (one stream slot each) */
+    double mean_batch;           /* requests per batch */
+    double p50_us, p99_us, max_us, mean_us;   /* arrival -> completion */
+    double span_s;               /* first arrival -> last completion */
+    double req_per_s;            /* completed / span */
+    double gb_per_s;             /* algorithmic bytes (input + result) / span */
+    size_t peak_hbm_bytes;       /* arena bytes held at the peak */
+    uint64_t admission_waits;    /* requests that waited for budget */
+} faf_server_stats;
+
+faf_server_config faf_server_config_init(void);
+faf_server *faf_server_create(const faf_server_config *cfg);
+void faf_server_destroy(faf_server *s);     /* waits for work in flight */
+/* switch plain per-request service on or off (waits for work in flight) */
+int faf_server_set_plain(faf_server *s, int on);
+/* build and upload the plan of (kind, n) now rather than at first use */
+int faf_server_prepare(faf_server *s, faf_req_kind kind, size_t n);
+/* bytes of the result of one (kind, n) request, 0 if not servable */
+size_t faf_server_result_bytes(faf_server *s, faf_req_kind kind, size_t n);
+/* returns a ticket >= 0, or -1 (faf_get_error) */
+int64_t faf_server_submit(faf_server *s, faf_req_kind kind, size_t n, const float *in,
+                          uint32_t flags);
+int faf_server_poll(faf_server *s, int64_t ticket);   /* 1 done, 0 pending, -1 failed */
+int faf_server_wait(faf_server *s, int64_t ticket);   /* 0 done, -1 failed */
+/* device pointer of a completed result (NULL if pending, failed or freed) */
+const float *faf_server_result(faf_server *s, int64_t ticket, size_t *bytes);
+int faf_server_release(faf_server *s, int64_t ticket);   /* returns its memory */
+int faf_server_drain(faf_server *s);                      /* waits for every request */
+int faf_server_stats_get(faf_server *s, faf_server_stats *st);
+void faf_server_stats_reset(faf_server *s);   /* later stats cover later requests */
+/* Replays a request stream in real time from the calling thread: request i
+ * is submitted arrival_ns[i] after the call starts (arrival = that instant)
+ * with `flags`; returns after all complete.  latency_ns, if given, receives
+ * each request's arrival -> completion time. */
+int faf_server_replay(faf_server *s, const int *kinds, const size_t *ns,
+                      const float *const *ins, const uint64_t *arrival_ns, size_t count,
+                      uint32_t flags, uint64_t *latency_ns);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,6 +108,23 @@ def test_creation_fails_loudly_without_a_device():
     assert b"GPU runtime unavailable" in L.faf_get_error()
 
 
+def test_server_config_defaults_and_no_silent_cpu_path():
+    """Serving defaults follow the stated request mix; without a GPU the
+    server refuses to start instead of computing on the CPU."""
+    import torch
+    L = fa.load()
+    c = L.faf_server_config_init()
+    assert c.hbm_budget == 64 << 30 and c.stft_win == 512 and c.stft_hop == 128
+    assert c.cwt_voices == 8 and abs(c.cwt_octaves - 31.5 / 8) < 1e-15 and c.cwt_fs == 16000.0
+    assert c.plain == 0 and c.streams >= 1 and c.max_batch >= 1
+    assert L.faf_server_submit(None, 0, 64, None, 0) == -1
+    assert L.faf_server_result_bytes(None, 0, 64) == 0
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    assert not L.faf_server_create(C.byref(c))
+    assert b"GPU runtime unavailable" in L.faf_get_error()
+
+
 def test_struct_layout_matches_reference_header(tmp_path):
     """offsetof/sizeof of the public structs and enum values are identical to
     the reference header's (compiled side by side when the reference tree
