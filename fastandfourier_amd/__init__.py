@@ -108,7 +108,9 @@ class FafServerStats(Structure):
                 ("launches", C.c_uint64), ("mean_batch", c_double), ("p50_us", c_double),
                 ("p99_us", c_double), ("max_us", c_double), ("mean_us", c_double),
                 ("span_s", c_double), ("req_per_s", c_double), ("gb_per_s", c_double),
-                ("peak_hbm_bytes", c_size_t), ("admission_waits", C.c_uint64)]
+                ("peak_hbm_bytes", c_size_t), ("admission_waits", C.c_uint64),
+                ("long_requests", C.c_uint64), ("host_us_per_batch", c_double),
+                ("device_us_per_batch", c_double)]
 
 
 PS = c_void_p

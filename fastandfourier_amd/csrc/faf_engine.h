@@ -331,10 +331,11 @@ extern __shared__ float2 faf_smem[];
 /* batched C2C, LDS-resident                                           */
 /* ------------------------------------------------------------------ */
 
-template <bool INV, bool INTER>
-__global__ void k_c2c(fafd_fft P, int spb, const float *in_re, const float *in_im,
+template <bool INV, bool INTER, int MT>
+__global__ __launch_bounds__(MT) void k_c2c(const fafd_fft *Pd, int spb, const float *in_re, const float *in_im,
                       long in_dist, float *out_re, float *out_im, long out_dist,
                       long count, float scale) {
+    const fafd_fft &P = *Pd;
     Slot sl = slot_of(P.threads, spb, (long)blockIdx.x * spb, count);
     Lds lds{faf_smem + sl.sid * P.plane};
     const long sig = sl.active ? sl.sig : 0;
@@ -372,10 +373,11 @@ FAF_DEV void kiss_pre(float2 a, float2 b_raw, float2 tw, float2 &fk, float2 &fm)
     fm = make_float2(e.x - o.x, o.y - e.y);
 }
 
-template <bool INTER_OUT>
-__global__ void k_r2c(fafd_fft P, int spb, const float2 *post, const float *x,
+template <bool INTER_OUT, int MT>
+__global__ __launch_bounds__(MT) void k_r2c(const fafd_fft *Pd, int spb, const float2 *post, const float *x,
                       long in_dist, float *out_re, float *out_im, long out_dist, long count,
                       float scale) {
+    const fafd_fft &P = *Pd;
     Slot sl = slot_of(P.threads, spb, (long)blockIdx.x * spb, count);
     Lds lds{faf_smem + sl.sid * P.plane};
     const long sig = sl.active ? sl.sig : 0;
@@ -407,10 +409,11 @@ __global__ void k_r2c(fafd_fft P, int spb, const float2 *post, const float *x,
     }
 }
 
-template <bool INTER_IN>
-__global__ void k_c2r(fafd_fft P, int spb, const float2 *pre, const float *in_re,
+template <bool INTER_IN, int MT>
+__global__ __launch_bounds__(MT) void k_c2r(const fafd_fft *Pd, int spb, const float2 *pre, const float *in_re,
                       const float *in_im, long in_dist, float *y, long out_dist, long count,
                       float scale) {
+    const fafd_fft &P = *Pd;
     Slot sl = slot_of(P.threads, spb, (long)blockIdx.x * spb, count);
     Lds lds{faf_smem + sl.sid * P.plane};
     const long sig = sl.active ? sl.sig : 0;
@@ -464,9 +467,15 @@ FAF_DEV float2 apply_ops(const fafd_ops &ops, int k, float2 v) {
 }
 
 #ifndef FAF_ENGINE_NO_PLAIN_KERNELS
-__global__ void k_pipe(fafd_fft Pf, fafd_fft Pi, int spb, const float2 *post,
-                       const float2 *pre, fafd_ops ops, const float *x, long in_dist,
+template <int MT>
+__global__ __launch_bounds__(MT) void k_pipe(const fafd_fft *Pfd,
+                                              const fafd_fft *Pid, int spb,
+                                              const float2 *post, const float2 *pre,
+                                              const fafd_ops *opsd, const float *x, long in_dist,
                        float *y, long out_dist, long count, float scale) {
+    const fafd_fft &Pf = *Pfd;
+    const fafd_fft &Pi = *Pid;
+    const fafd_ops &ops = *opsd;
     Slot sl = slot_of(Pf.threads, spb, (long)blockIdx.x * spb, count);
     Lds lds{faf_smem + sl.sid * Pf.plane};
     const long sig = sl.active ? sl.sig : 0;
@@ -521,10 +530,12 @@ struct GFrameIn {            /* windowed frame packed as z[i] = (w x)[2i] + i (w
 };
 
 #ifndef FAF_ENGINE_NO_PLAIN_KERNELS
-__global__ void k_stft(fafd_fft P, int spb, const float2 *post, const float *win,
+template <int MT>
+__global__ __launch_bounds__(MT) void k_stft(const fafd_fft *Pd, int spb, const float2 *post, const float *win,
                        const float *x, long rec_len, long rec_dist, long hop, long frames,
                        float *out_re, float *out_im, long frame_dist, long rec_out_dist,
                        long count) {
+    const fafd_fft &P = *Pd;
     Slot sl = slot_of(P.threads, spb, (long)blockIdx.x * spb, count);
     Lds lds{faf_smem + sl.sid * P.plane};
     const long sig = sl.active ? sl.sig : 0;
@@ -572,10 +583,13 @@ struct OneSidedIn {          /* Z[k] from the packed spectrum X and a real filte
     }
 };
 
-template <bool REAL_OUT>
-__global__ void k_cwt(fafd_fft Ph, fafd_fft Pn, const float2 *post, const float *psi,
+template <bool REAL_OUT, int MT>
+__global__ __launch_bounds__(MT) void k_cwt(const fafd_fft *Phd,
+                                              const fafd_fft *Pnd, const float2 *post, const float *psi,
                       long psi_stride, int rows, int rows_per_wg, const float *x,
                       long in_dist, float *out, long out_dist, long count, float scale) {
+    const fafd_fft &Ph = *Phd;
+    const fafd_fft &Pn = *Pnd;
     /* one segment per workgroup; blockIdx.x = seg * groups + group */
     const int groups = (rows + rows_per_wg - 1) / rows_per_wg;
     const long seg = blockIdx.x / groups;
@@ -625,9 +639,13 @@ __global__ void k_cwt(fafd_fft Ph, fafd_fft Pn, const float2 *post, const float 
  * acc = sum_r rfft(W_r) * dual_r, then y = irfft(acc), in one workgroup.
  * X holds the packed row spectrum, A the accumulator (nc+1 bins). */
 #ifndef FAF_ENGINE_NO_PLAIN_KERNELS
-__global__ void k_icwt(fafd_fft Pf, fafd_fft Pi, const float2 *post, const float2 *pre,
+template <int MT>
+__global__ __launch_bounds__(MT) void k_icwt(const fafd_fft *Pfd,
+                                              const fafd_fft *Pid, const float2 *post, const float2 *pre,
                        const float *dual, long dual_stride, int rows, const float *w,
                        long in_dist, float *y, long out_dist, long count, float scale) {
+    const fafd_fft &Pf = *Pfd;
+    const fafd_fft &Pi = *Pid;
     const long seg = blockIdx.x;
     const int tid = (int)threadIdx.x, T = Pf.threads, nc = Pf.n;
     const int active = seg < count;
@@ -688,10 +706,11 @@ enum { FS_IN_SPLIT = 0, FS_IN_PAIR = 1, FS_IN_INTER = 2 };
 /* output kinds of pass B */
 enum { FS_OUT_SPLIT = 0, FS_OUT_PAIR = 1, FS_OUT_INTER = 2 };
 
-template <bool INV, int IN>
-__global__ void k_fs_col(fafd_fft P1, int n2, int cols, const float2 *twn,
+template <bool INV, int IN, int MT>
+__global__ __launch_bounds__(MT) void k_fs_col(const fafd_fft *P1d, int n2, int cols, const float2 *twn,
                          const float *in_re, const float *in_im, long in_dist,
                          float *t_re, float *t_im, long t_dist) {
+    const fafd_fft &P1 = *P1d;
     const long b = blockIdx.y;
     const int c0 = blockIdx.x * cols;
     const int n1 = P1.n, tid = (int)threadIdx.x, bd = (int)blockDim.x;
@@ -724,10 +743,11 @@ __global__ void k_fs_col(fafd_fft P1, int n2, int cols, const float2 *twn,
     }
 }
 
-template <bool INV, int OUT>
-__global__ void k_fs_row(fafd_fft P2, int n1, int rows, const float *t_re,
+template <bool INV, int OUT, int MT>
+__global__ __launch_bounds__(MT) void k_fs_row(const fafd_fft *P2d, int n1, int rows, const float *t_re,
                          const float *t_im, long t_dist, float *out_re, float *out_im,
                          long out_dist, float scale) {
+    const fafd_fft &P2 = *P2d;
     const long b = blockIdx.y;
     const int k0 = blockIdx.x * rows;
     const int n2 = P2.n, tid = (int)threadIdx.x, bd = (int)blockDim.x;
@@ -899,9 +919,13 @@ FAF_DEV void blue_body(const fafd_fft &Pf, const fafd_fft &Pi, const fafd_blue &
 }
 
 #ifndef FAF_ENGINE_NO_PLAIN_KERNELS
-__global__ void k_rader(fafd_fft Pf, fafd_fft Pi, fafd_rader R, int spb, const float *in_re,
+template <int MT>
+__global__ __launch_bounds__(MT) void k_rader(const fafd_fft *Pfd,
+                                               const fafd_fft *Pid, fafd_rader R, int spb, const float *in_re,
                         const float *in_im, long in_dist, float *out_re, float *out_im,
                         long out_dist, long count, float scale) {
+    const fafd_fft &Pf = *Pfd;
+    const fafd_fft &Pi = *Pid;
     Slot sl = slot_of(Pf.threads, spb, (long)blockIdx.x * spb, count);
     const int per = Pf.plane + lpad(R.p) + 1;
     Lds xa{faf_smem + sl.sid * per};
@@ -915,9 +939,13 @@ __global__ void k_rader(fafd_fft Pf, fafd_fft Pi, fafd_rader R, int spb, const f
 
 
 #ifndef FAF_ENGINE_NO_PLAIN_KERNELS
-__global__ void k_blue(fafd_fft Pf, fafd_fft Pi, fafd_blue Bq, int spb, const float *in_re,
+template <int MT>
+__global__ __launch_bounds__(MT) void k_blue(const fafd_fft *Pfd,
+                                              const fafd_fft *Pid, fafd_blue Bq, int spb, const float *in_re,
                        const float *in_im, long in_dist, float *out_re, float *out_im,
                        long out_dist, long count, float scale) {
+    const fafd_fft &Pf = *Pfd;
+    const fafd_fft &Pi = *Pid;
     Slot sl = slot_of(Pf.threads, spb, (long)blockIdx.x * spb, count);
     Lds aa{faf_smem + sl.sid * Pf.plane};
     const long sig = sl.active ? sl.sig : 0;
@@ -934,7 +962,8 @@ __global__ void k_blue(fafd_fft Pf, fafd_fft Pi, fafd_blue Bq, int spb, const fl
 /* ------------------------------------------------------------------ */
 
 #ifndef FAF_ENGINE_NO_PLAIN_KERNELS
-__global__ void k_ragged(const fafd_rplan *plans, const fafd_unit *units, int nunits,
+template <int MT>
+__global__ __launch_bounds__(MT) void k_ragged(const fafd_rplan *plans, const fafd_unit *units, int nunits,
                          const int32_t *order, const long *offsets, const float *in_re,
                          const float *in_im, float *out_re, float *out_im,
                          unsigned *counter) {
@@ -1018,7 +1047,8 @@ FAF_DEV void post_pass_inter(const Lds &z, const float2 *post, int nc, int lt, i
 }
 
 #ifndef FAF_ENGINE_NO_PLAIN_KERNELS
-__global__ void k_serve(const fafd_splan *plans, const fafd_sunit *units, int nunits,
+template <int MT>
+__global__ __launch_bounds__(MT) void k_serve(const fafd_splan *plans, const fafd_sunit *units, int nunits,
                         const fafd_ssig *sigs, unsigned *counter) {
     /* with a counter, workgroups pull units from it until the table is
      * drained; without one, workgroup b runs unit b only */

@@ -85,6 +85,7 @@ int fafk_cwt_spread(const float *X, long x_dist, const float *psi, long psi_stri
                     int analytic, float *out, long row_len, long seg_dist, int rows,
                     long count, void *stream);   /* dists in complex elements */
 void *fafk_host_alloc(size_t bytes);     /* pinned */
+const void *fafk_plan_record(const fafd_fft *P);   /* device copy, uploaded once */
 void fafk_host_free(void *p);
 
 /* compile-time specialised kernels (faf_kernels_ct.hip): power-of-two

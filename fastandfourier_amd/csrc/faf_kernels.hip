@@ -1,12 +1,15 @@
 /*
  * faf_kernels.hip -- the thin C-ABI layer between the C11 host code and the
  * gfx950 kernels of faf_engine.h.  Every function takes plain pointers,
- * sizes and plan records by value, launches on the given stream and
- * returns 0 or a negative code (message in fafk_last_error()).
+ * sizes and host copies of the plan records, launches on the given stream
+ * and returns 0 or a negative code (message in fafk_last_error()).
  */
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
+#include <mutex>
+#include <string>
+#include <unordered_map>
 #include "faf_engine.h"
 #include "faf_kern.h"
 
@@ -39,6 +42,17 @@ template <class K> static void lds_attr(K k) {
     (void)hipGetLastError();
 }
 
+template <int M> static void init_attrs() {
+    lds_attr(k_c2c<false, false, M>); lds_attr(k_c2c<true, false, M>);
+    lds_attr(k_c2c<false, true, M>);  lds_attr(k_c2c<true, true, M>);
+    lds_attr(k_r2c<false, M>); lds_attr(k_r2c<true, M>);
+    lds_attr(k_c2r<false, M>); lds_attr(k_c2r<true, M>);
+    lds_attr(k_pipe<M>); lds_attr(k_stft<M>);
+    lds_attr(k_cwt<false, M>); lds_attr(k_cwt<true, M>);
+    lds_attr(k_rader<M>); lds_attr(k_blue<M>); lds_attr(k_ragged<M>); lds_attr(k_icwt<M>);
+    lds_attr(k_serve<M>);
+}
+
 extern "C" int fafk_init(void) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
@@ -47,20 +61,15 @@ extern "C" int fafk_init(void) {
                  e == hipSuccess ? "count 0" : hipGetErrorString(e));
         return -1;
     }
-    lds_attr(k_c2c<false, false>); lds_attr(k_c2c<true, false>);
-    lds_attr(k_c2c<false, true>);  lds_attr(k_c2c<true, true>);
-    lds_attr(k_r2c<false>); lds_attr(k_r2c<true>);
-    lds_attr(k_c2r<false>); lds_attr(k_c2r<true>);
-    lds_attr(k_pipe); lds_attr(k_stft);
-    lds_attr(k_cwt<false>); lds_attr(k_cwt<true>);
-    lds_attr(k_fs_col<false, FS_IN_SPLIT>); lds_attr(k_fs_col<true, FS_IN_SPLIT>);
-    lds_attr(k_fs_col<false, FS_IN_PAIR>);  lds_attr(k_fs_col<true, FS_IN_PAIR>);
-    lds_attr(k_fs_col<false, FS_IN_INTER>); lds_attr(k_fs_col<true, FS_IN_INTER>);
-    lds_attr(k_fs_row<false, FS_OUT_SPLIT>); lds_attr(k_fs_row<true, FS_OUT_SPLIT>);
-    lds_attr(k_fs_row<false, FS_OUT_PAIR>);  lds_attr(k_fs_row<true, FS_OUT_PAIR>);
-    lds_attr(k_fs_row<false, FS_OUT_INTER>); lds_attr(k_fs_row<true, FS_OUT_INTER>);
-    lds_attr(k_rader); lds_attr(k_blue); lds_attr(k_ragged); lds_attr(k_icwt);
-    lds_attr(k_serve);
+    init_attrs<512>();
+    init_attrs<1024>();
+    constexpr int M = 1024;
+    lds_attr(k_fs_col<false, FS_IN_SPLIT, M>); lds_attr(k_fs_col<true, FS_IN_SPLIT, M>);
+    lds_attr(k_fs_col<false, FS_IN_PAIR, M>);  lds_attr(k_fs_col<true, FS_IN_PAIR, M>);
+    lds_attr(k_fs_col<false, FS_IN_INTER, M>); lds_attr(k_fs_col<true, FS_IN_INTER, M>);
+    lds_attr(k_fs_row<false, FS_OUT_SPLIT, M>); lds_attr(k_fs_row<true, FS_OUT_SPLIT, M>);
+    lds_attr(k_fs_row<false, FS_OUT_PAIR, M>);  lds_attr(k_fs_row<true, FS_OUT_PAIR, M>);
+    lds_attr(k_fs_row<false, FS_OUT_INTER, M>); lds_attr(k_fs_row<true, FS_OUT_INTER, M>);
     return 0;
 }
 
@@ -188,6 +197,83 @@ extern "C" int fafk_window_inter(const float *x, long in_dist, const float *w, l
 
 /* ---------------- transform launches ---------------- */
 
+/* Plan records live in device memory: a kernel indexes P.st[s] with a
+ * runtime stage number, which on a by-value kernel argument makes the
+ * compiler materialise the whole record in registers and scratch.  Records
+ * are immutable, so each distinct one is uploaded once (keyed by its bytes)
+ * into 64 KB slabs that live until the process ends. */
+static std::mutex g_rec_mu;
+static std::unordered_map<std::string, const void *> g_recs;
+static char *g_slab;
+static size_t g_slab_left;
+
+static const void *dev_rec(const void *h, size_t bytes) {
+    std::string key((const char *)h, bytes);
+    std::lock_guard<std::mutex> lk(g_rec_mu);
+    auto it = g_recs.find(key);
+    if (it != g_recs.end()) return it->second;
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (g_slab_left < need) {
+        const size_t sz = need > 65536 ? need : 65536;
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, sz);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            fail("hipMalloc (plan records)", e);
+            return nullptr;
+        }
+        g_slab = (char *)p;
+        g_slab_left = sz;
+    }
+    char *d = g_slab;
+    hipError_t e = hipMemcpy(d, h, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail("hipMemcpy (plan records)", e);
+        return nullptr;
+    }
+    g_slab += need;
+    g_slab_left -= need;
+    g_recs.emplace(std::move(key), d);
+    return d;
+}
+
+static const fafd_fft *dev_fft(const fafd_fft *P) {
+    return (const fafd_fft *)dev_rec(P, sizeof *P);
+}
+
+/* the device copy of a spectral-op list, built from its fields so padding
+ * bytes never split the cache */
+static const fafd_ops *dev_ops(const fafd_ops *o) {
+    fafd_ops c;
+    memset(&c, 0, sizeof c);
+    c.nops = o->nops;
+    for (int i = 0; i < FAFD_MAX_OPS; ++i) {
+        c.code[i] = o->code[i];
+        c.a1[i] = o->a1[i];
+        c.a2[i] = o->a2[i];
+    }
+    c.h_re = o->h_re;
+    c.h_im = o->h_im;
+    c.h_bins = o->h_bins;
+    return (const fafd_ops *)dev_rec(&c, sizeof c);
+}
+
+/* upload a plan record ahead of its first launch (plan creation time) */
+extern "C" const void *fafk_plan_record(const fafd_fft *P) { return dev_fft(P); }
+
+#define NEED(p) \
+    do { if (!(p)) { snprintf(g_err, sizeof g_err, "device plan record upload failed"); return -1; } } while (0)
+
+/* kernels compiled for at most 512 threads may use up to 256 VGPRs and so
+ * never spill; 1024-thread launches (lengths whose plan needs more than 512
+ * threads) take the 1024 variant */
+#define BY_MT(bx, ...)                                                    \
+    do {                                                                  \
+        if ((bx) <= 512) { constexpr int MT = 512; __VA_ARGS__; }         \
+        else { constexpr int MT = 1024; __VA_ARGS__; }                    \
+    } while (0)
+
 static inline unsigned blocks_for(long count, int spb) {
     return (unsigned)((count + spb - 1) / spb);
 }
@@ -197,13 +283,12 @@ extern "C" int fafk_c2c(const fafd_fft *P, int inv, int inter, int spb, const fl
                         long out_dist, long count, float scale, void *stream) {
     dim3 g(blocks_for(count, spb)), b(P->threads * spb);
     size_t lds = (size_t)spb * P->plane * sizeof(float2);
-    if (inter) {
-        if (inv) hipLaunchKernelGGL((k_c2c<true, true>), g, b, lds, S(stream), *P, spb, in_re, in_im, in_dist, out_re, out_im, out_dist, count, scale);
-        else hipLaunchKernelGGL((k_c2c<false, true>), g, b, lds, S(stream), *P, spb, in_re, in_im, in_dist, out_re, out_im, out_dist, count, scale);
-    } else {
-        if (inv) hipLaunchKernelGGL((k_c2c<true, false>), g, b, lds, S(stream), *P, spb, in_re, in_im, in_dist, out_re, out_im, out_dist, count, scale);
-        else hipLaunchKernelGGL((k_c2c<false, false>), g, b, lds, S(stream), *P, spb, in_re, in_im, in_dist, out_re, out_im, out_dist, count, scale);
-    }
+    const fafd_fft *Pd = dev_fft(P);
+    NEED(Pd);
+#define C2C(I, X) hipLaunchKernelGGL((k_c2c<I, X, MT>), g, b, lds, S(stream), Pd, spb, in_re, in_im, in_dist, out_re, out_im, out_dist, count, scale)
+    BY_MT(b.x, if (inter) { if (inv) C2C(true, true); else C2C(false, true); }
+               else { if (inv) C2C(true, false); else C2C(false, false); });
+#undef C2C
     LAUNCH_CHECK("k_c2c");
     return 0;
 }
@@ -213,10 +298,11 @@ extern "C" int fafk_r2c(const fafd_fft *P, int inter_out, int spb, const void *p
                         long out_dist, long count, float scale, void *stream) {
     dim3 g(blocks_for(count, spb)), b(P->threads * spb);
     size_t lds = (size_t)spb * P->plane * sizeof(float2);
-    if (inter_out)
-        hipLaunchKernelGGL((k_r2c<true>), g, b, lds, S(stream), *P, spb, (const float2 *)post, x, in_dist, out_re, out_im, out_dist, count, scale);
-    else
-        hipLaunchKernelGGL((k_r2c<false>), g, b, lds, S(stream), *P, spb, (const float2 *)post, x, in_dist, out_re, out_im, out_dist, count, scale);
+    const fafd_fft *Pd = dev_fft(P);
+    NEED(Pd);
+#define R2C(X) hipLaunchKernelGGL((k_r2c<X, MT>), g, b, lds, S(stream), Pd, spb, (const float2 *)post, x, in_dist, out_re, out_im, out_dist, count, scale)
+    BY_MT(b.x, if (inter_out) R2C(true); else R2C(false));
+#undef R2C
     LAUNCH_CHECK("k_r2c");
     return 0;
 }
@@ -226,10 +312,11 @@ extern "C" int fafk_c2r(const fafd_fft *P, int inter_in, int spb, const void *pr
                         long out_dist, long count, float scale, void *stream) {
     dim3 g(blocks_for(count, spb)), b(P->threads * spb);
     size_t lds = (size_t)spb * P->plane * sizeof(float2);
-    if (inter_in)
-        hipLaunchKernelGGL((k_c2r<true>), g, b, lds, S(stream), *P, spb, (const float2 *)pre, in_re, in_im, in_dist, y, out_dist, count, scale);
-    else
-        hipLaunchKernelGGL((k_c2r<false>), g, b, lds, S(stream), *P, spb, (const float2 *)pre, in_re, in_im, in_dist, y, out_dist, count, scale);
+    const fafd_fft *Pd = dev_fft(P);
+    NEED(Pd);
+#define C2R(X) hipLaunchKernelGGL((k_c2r<X, MT>), g, b, lds, S(stream), Pd, spb, (const float2 *)pre, in_re, in_im, in_dist, y, out_dist, count, scale)
+    BY_MT(b.x, if (inter_in) C2R(true); else C2R(false));
+#undef C2R
     LAUNCH_CHECK("k_c2r");
     return 0;
 }
@@ -239,8 +326,12 @@ extern "C" int fafk_pipe(const fafd_fft *Pf, const fafd_fft *Pi, int spb, const 
                          float *y, long out_dist, long count, float scale, void *stream) {
     dim3 g(blocks_for(count, spb)), b(Pf->threads * spb);
     size_t lds = (size_t)spb * Pf->plane * sizeof(float2);
-    hipLaunchKernelGGL(k_pipe, g, b, lds, S(stream), *Pf, *Pi, spb, (const float2 *)post,
-                       (const float2 *)pre, *ops, x, in_dist, y, out_dist, count, scale);
+    const fafd_fft *Pfd = dev_fft(Pf), *Pid = dev_fft(Pi);
+    const fafd_ops *od = dev_ops(ops);
+    NEED(Pfd && Pid && od);
+    BY_MT(b.x, hipLaunchKernelGGL(k_pipe<MT>, g, b, lds, S(stream), Pfd, Pid, spb,
+                                  (const float2 *)post, (const float2 *)pre, od, x, in_dist, y,
+                                  out_dist, count, scale));
     LAUNCH_CHECK("k_pipe");
     return 0;
 }
@@ -251,9 +342,11 @@ extern "C" int fafk_stft(const fafd_fft *P, int spb, const void *post, const flo
                          long count, void *stream) {
     dim3 g(blocks_for(count, spb)), b(P->threads * spb);
     size_t lds = (size_t)spb * P->plane * sizeof(float2);
-    hipLaunchKernelGGL(k_stft, g, b, lds, S(stream), *P, spb, (const float2 *)post, win, x,
-                       rec_len, rec_dist, hop, frames, out_re, out_im, frame_dist,
-                       rec_out_dist, count);
+    const fafd_fft *Pd = dev_fft(P);
+    NEED(Pd);
+    BY_MT(b.x, hipLaunchKernelGGL(k_stft<MT>, g, b, lds, S(stream), Pd, spb,
+                                  (const float2 *)post, win, x, rec_len, rec_dist, hop, frames,
+                                  out_re, out_im, frame_dist, rec_out_dist, count));
     LAUNCH_CHECK("k_stft");
     return 0;
 }
@@ -266,10 +359,11 @@ extern "C" int fafk_cwt(const fafd_fft *Ph, const fafd_fft *Pn, int real_out, co
     dim3 g((unsigned)(count * groups));
     dim3 b(Ph->threads > Pn->threads ? Ph->threads : Pn->threads);
     size_t lds = (size_t)(Ph->plane + Pn->plane) * sizeof(float2);
-    if (real_out)
-        hipLaunchKernelGGL((k_cwt<true>), g, b, lds, S(stream), *Ph, *Pn, (const float2 *)post, psi, psi_stride, rows, rows_per_wg, x, in_dist, out, out_dist, count, scale);
-    else
-        hipLaunchKernelGGL((k_cwt<false>), g, b, lds, S(stream), *Ph, *Pn, (const float2 *)post, psi, psi_stride, rows, rows_per_wg, x, in_dist, out, out_dist, count, scale);
+    const fafd_fft *Phd = dev_fft(Ph), *Pnd = dev_fft(Pn);
+    NEED(Phd && Pnd);
+#define CWT(R) hipLaunchKernelGGL((k_cwt<R, MT>), g, b, lds, S(stream), Phd, Pnd, (const float2 *)post, psi, psi_stride, rows, rows_per_wg, x, in_dist, out, out_dist, count, scale)
+    BY_MT(b.x, if (real_out) CWT(true); else CWT(false));
+#undef CWT
     LAUNCH_CHECK("k_cwt");
     return 0;
 }
@@ -280,9 +374,11 @@ extern "C" int fafk_icwt(const fafd_fft *Pf, const fafd_fft *Pi, const void *pos
                          float scale, void *stream) {
     dim3 g((unsigned)count), b(Pf->threads);
     size_t lds = (size_t)2 * Pf->plane * sizeof(float2);
-    hipLaunchKernelGGL(k_icwt, g, b, lds, S(stream), *Pf, *Pi, (const float2 *)post,
-                       (const float2 *)pre, dual, dual_stride, rows, w, in_dist, y, out_dist,
-                       count, scale);
+    const fafd_fft *Pfd = dev_fft(Pf), *Pid = dev_fft(Pi);
+    NEED(Pfd && Pid);
+    BY_MT(b.x, hipLaunchKernelGGL(k_icwt<MT>, g, b, lds, S(stream), Pfd, Pid,
+                                  (const float2 *)post, (const float2 *)pre, dual, dual_stride,
+                                  rows, w, in_dist, y, out_dist, count, scale));
     LAUNCH_CHECK("k_icwt");
     return 0;
 }
@@ -294,16 +390,24 @@ extern "C" int fafk_fs_col(const fafd_fft *P1, int inv, int in_kind, int n2, int
     dim3 g((unsigned)((n2 + cols - 1) / cols), (unsigned)count), b(P1->threads * cols);
     size_t lds = (size_t)cols * P1->plane * sizeof(float2);
     const float2 *tw = (const float2 *)twn;
-#define FSC(I, K) hipLaunchKernelGGL((k_fs_col<I, K>), g, b, lds, S(stream), *P1, n2, cols, tw, in_re, in_im, in_dist, t_re, t_im, t_dist)
-    if (inv) {
-        if (in_kind == FS_IN_SPLIT) FSC(true, FS_IN_SPLIT);
-        else if (in_kind == FS_IN_PAIR) FSC(true, FS_IN_PAIR);
-        else FSC(true, FS_IN_INTER);
-    } else {
-        if (in_kind == FS_IN_SPLIT) FSC(false, FS_IN_SPLIT);
-        else if (in_kind == FS_IN_PAIR) FSC(false, FS_IN_PAIR);
-        else FSC(false, FS_IN_INTER);
-    }
+    const fafd_fft *Pd = dev_fft(P1);
+    NEED(Pd);
+    /* the strided column and row passes hide HBM latency with occupancy:
+     * the 128-register variant (four waves per SIMD) beats the spill-free
+     * 512-thread one here */
+    constexpr int MT = 1024;
+#define FSC(I, K) hipLaunchKernelGGL((k_fs_col<I, K, MT>), g, b, lds, S(stream), Pd, n2, cols, tw, in_re, in_im, in_dist, t_re, t_im, t_dist)
+    do {
+          if (inv) {
+              if (in_kind == FS_IN_SPLIT) FSC(true, FS_IN_SPLIT);
+              else if (in_kind == FS_IN_PAIR) FSC(true, FS_IN_PAIR);
+              else FSC(true, FS_IN_INTER);
+          } else {
+              if (in_kind == FS_IN_SPLIT) FSC(false, FS_IN_SPLIT);
+              else if (in_kind == FS_IN_PAIR) FSC(false, FS_IN_PAIR);
+              else FSC(false, FS_IN_INTER);
+          }
+    } while (0);
 #undef FSC
     LAUNCH_CHECK("k_fs_col");
     return 0;
@@ -315,16 +419,21 @@ extern "C" int fafk_fs_row(const fafd_fft *P2, int inv, int out_kind, int n1, in
                            void *stream) {
     dim3 g((unsigned)((n1 + rows - 1) / rows), (unsigned)count), b(P2->threads * rows);
     size_t lds = (size_t)rows * P2->plane * sizeof(float2);
-#define FSR(I, K) hipLaunchKernelGGL((k_fs_row<I, K>), g, b, lds, S(stream), *P2, n1, rows, t_re, t_im, t_dist, out_re, out_im, out_dist, scale)
-    if (inv) {
-        if (out_kind == FS_OUT_SPLIT) FSR(true, FS_OUT_SPLIT);
-        else if (out_kind == FS_OUT_PAIR) FSR(true, FS_OUT_PAIR);
-        else FSR(true, FS_OUT_INTER);
-    } else {
-        if (out_kind == FS_OUT_SPLIT) FSR(false, FS_OUT_SPLIT);
-        else if (out_kind == FS_OUT_PAIR) FSR(false, FS_OUT_PAIR);
-        else FSR(false, FS_OUT_INTER);
-    }
+    const fafd_fft *Pd = dev_fft(P2);
+    NEED(Pd);
+    constexpr int MT = 1024;     /* as for the column pass */
+#define FSR(I, K) hipLaunchKernelGGL((k_fs_row<I, K, MT>), g, b, lds, S(stream), Pd, n1, rows, t_re, t_im, t_dist, out_re, out_im, out_dist, scale)
+    do {
+          if (inv) {
+              if (out_kind == FS_OUT_SPLIT) FSR(true, FS_OUT_SPLIT);
+              else if (out_kind == FS_OUT_PAIR) FSR(true, FS_OUT_PAIR);
+              else FSR(true, FS_OUT_INTER);
+          } else {
+              if (out_kind == FS_OUT_SPLIT) FSR(false, FS_OUT_SPLIT);
+              else if (out_kind == FS_OUT_PAIR) FSR(false, FS_OUT_PAIR);
+              else FSR(false, FS_OUT_INTER);
+          }
+    } while (0);
 #undef FSR
     LAUNCH_CHECK("k_fs_row");
     return 0;
@@ -361,8 +470,10 @@ extern "C" int fafk_rader(const fafd_fft *Pf, const fafd_fft *Pi, const fafd_rad
     dim3 g(blocks_for(count, spb)), b(Pf->threads * spb);
     const int per = Pf->plane + R->p + (R->p >> 4) + 1;
     size_t lds = (size_t)spb * per * sizeof(float2);
-    hipLaunchKernelGGL(k_rader, g, b, lds, S(stream), *Pf, *Pi, *R, spb, in_re, in_im,
-                       in_dist, out_re, out_im, out_dist, count, scale);
+    const fafd_fft *Pfd = dev_fft(Pf), *Pid = dev_fft(Pi);
+    NEED(Pfd && Pid);
+    BY_MT(b.x, hipLaunchKernelGGL(k_rader<MT>, g, b, lds, S(stream), Pfd, Pid, *R, spb, in_re,
+                                  in_im, in_dist, out_re, out_im, out_dist, count, scale));
     LAUNCH_CHECK("k_rader");
     return 0;
 }
@@ -372,8 +483,10 @@ extern "C" int fafk_blue(const fafd_fft *Pf, const fafd_fft *Pi, const fafd_blue
                          float *out_im, long out_dist, long count, float scale, void *stream) {
     dim3 g(blocks_for(count, spb)), b(Pf->threads * spb);
     size_t lds = (size_t)spb * Pf->plane * sizeof(float2);
-    hipLaunchKernelGGL(k_blue, g, b, lds, S(stream), *Pf, *Pi, *Bq, spb, in_re, in_im,
-                       in_dist, out_re, out_im, out_dist, count, scale);
+    const fafd_fft *Pfd = dev_fft(Pf), *Pid = dev_fft(Pi);
+    NEED(Pfd && Pid);
+    BY_MT(b.x, hipLaunchKernelGGL(k_blue<MT>, g, b, lds, S(stream), Pfd, Pid, *Bq, spb, in_re,
+                                  in_im, in_dist, out_re, out_im, out_dist, count, scale));
     LAUNCH_CHECK("k_blue");
     return 0;
 }
@@ -385,9 +498,9 @@ extern "C" int fafk_ragged(const void *plans, const void *units, int nunits,
                            void *stream) {
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, S(stream), counter, 1, 0u);
     LAUNCH_CHECK("k_fill_u32");
-    hipLaunchKernelGGL(k_ragged, dim3(grid), dim3(block), lds, S(stream),
-                       (const fafd_rplan *)plans, (const fafd_unit *)units, nunits, order,
-                       offsets, in_re, in_im, out_re, out_im, counter);
+    BY_MT(block, hipLaunchKernelGGL(k_ragged<MT>, dim3(grid), dim3(block), lds, S(stream),
+                                    (const fafd_rplan *)plans, (const fafd_unit *)units, nunits,
+                                    order, offsets, in_re, in_im, out_re, out_im, counter));
     LAUNCH_CHECK("k_ragged");
     return 0;
 }
@@ -401,9 +514,9 @@ extern "C" int fafk_serve(const void *plans, const void *units, int nunits, cons
     } else {
         grid = nunits;
     }
-    hipLaunchKernelGGL(k_serve, dim3(grid), dim3(block), lds, S(stream),
-                       (const fafd_splan *)plans, (const fafd_sunit *)units, nunits,
-                       (const fafd_ssig *)sigs, counter);
+    BY_MT(block, hipLaunchKernelGGL(k_serve<MT>, dim3(grid), dim3(block), lds, S(stream),
+                                    (const fafd_splan *)plans, (const fafd_sunit *)units,
+                                    nunits, (const fafd_ssig *)sigs, counter));
     LAUNCH_CHECK("k_serve");
     return 0;
 }

@@ -154,7 +154,14 @@ int fafi_make_fft(size_t n, int inv, int allow7, const int *order, int norder, f
         return -1;
     }
     P->tw = fafi_twiddles(n, inv);
-    return P->tw ? 0 : -1;
+    if (!P->tw) return -1;
+    /* the kernels read the record from device memory: upload it now rather
+     * than at the first launch */
+    if (!fafk_plan_record(P)) {
+        fafi_err("%s", fafk_last_error());
+        return -1;
+    }
+    return 0;
 }
 
 int fafi_default_spb(const fafd_fft *P, size_t extra_per_signal) {

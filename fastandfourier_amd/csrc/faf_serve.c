@@ -39,6 +39,7 @@
 #define SV_MAX_SLOTS 8
 #define SV_CWT_ROWS_PER_UNIT 4
 #define SV_BLOCK_MAX 1024
+#define SV_BLOCK_SMALL 512
 #define SV_ALIGN 256
 
 enum { ST_QUEUED = 0, ST_ADMITTED, ST_LAUNCHED, ST_DONE, ST_FAILED };
@@ -78,6 +79,7 @@ typedef struct srec {
 typedef struct slot {
     void *stream, *ev_small, *ev_done;
     int busy, nsmall, nreq, small_done, cap;
+    uint64_t t_launched;
     int64_t *reqs;
     void *tab;               /* pinned: units then signals */
     size_t tab_bytes;
@@ -130,10 +132,11 @@ struct faf_server {
     fafd_sunit *units;
     double *ucost;
     size_t units_cap;
-    int *uorder;
+    int *uorder, *ulds;
     /* stats */
     int64_t stats_first;
-    _Atomic uint64_t launches, launched_reqs, admission_waits;
+    _Atomic uint64_t launches, launched_reqs, admission_waits, long_reqs;
+    _Atomic uint64_t host_ns, device_ns;     /* per batch: build + launch, launch -> done */
     char err[256];
 };
 
@@ -267,7 +270,9 @@ static int shape_fits(sentry *e) {
     const size_t cap = FAFD_MAX_LDS_BYTES - SV_ALIGN;
     size_t per = (size_t)e->lds_per * 8;
     if (per > cap) return 0;
-    int by_t = SV_BLOCK_MAX / e->T;
+    /* units stay within 512 threads (the serving kernel's register budget
+     * without spills) unless one signal alone needs more */
+    int by_t = e->T <= SV_BLOCK_SMALL ? SV_BLOCK_SMALL / e->T : 1;
     int by_l = (int)(cap / per);
     e->spb = by_t < by_l ? by_t : by_l;
     return e->spb >= 1;
@@ -569,6 +574,7 @@ static int run_long(faf_server *s, srec *r, void *st) {
  * the slot's stream has drained), or -2 when nothing was popped. */
 static int launch(faf_server *s, slot *sl, int take) {
     const uint64_t stamp = ++s->stamp;
+    const uint64_t t0 = now_ns();
     if (sl->cap < take) {
         int64_t *v = realloc(sl->reqs, (size_t)take * sizeof(int64_t));
         if (!v) return -2;
@@ -659,9 +665,9 @@ static int launch(faf_server *s, slot *sl, int take) {
         if (grow((void **)&s->units, &s->units_cap, nunits, sizeof(fafd_sunit))) return -1;
         s->ucost = realloc(s->ucost, s->units_cap * sizeof(double));
         s->uorder = realloc(s->uorder, s->units_cap * sizeof(int));
-        if (!s->ucost || !s->uorder) return -1;
+        s->ulds = realloc(s->ulds, s->units_cap * sizeof(int));
+        if (!s->ucost || !s->uorder || !s->ulds) return -1;
         size_t u = 0;
-        int block = 64, lds = 0;
         for (int b = 0, first = 0; b < nb; ++b) {
             sentry *e = s->bk_e[b];
             for (int i = 0; i < s->bk_cnt[b]; i += e->spb, ++u) {
@@ -669,12 +675,10 @@ static int launch(faf_server *s, slot *sl, int take) {
                 s->units[u].plan = e->splan;
                 s->units[u].count = c;
                 s->units[u].first = first + i;
-                s->units[u].pad = 0;
+                s->units[u].pad = c * e->T;          /* threads it needs */
                 s->ucost[u] = e->cost * (double)c;
                 s->uorder[u] = (int)u;
-                int th = (c * e->T + 63) & ~63;
-                if (th > block) block = th;
-                if (c * e->lds_per * 8 > lds) lds = c * e->lds_per * 8;
+                s->ulds[u] = c * e->lds_per * 8;
             }
             first += s->bk_cnt[b];
         }
@@ -689,11 +693,30 @@ static int launch(faf_server *s, slot *sl, int take) {
             sl->tab_bytes = sl->tab ? nb2 : 0;
             if (!sl->tab) { set_err(s, fafk_last_error()); return -1; }
         }
+        /* units wider than 512 threads go first, in a launch of their own
+         * with 1024-thread workgroups; the rest share one 512-wide launch */
         fafd_sunit *tu = sl->tab;
         fafd_ssig *ts = (fafd_ssig *)(tu + nunits);
-        for (size_t i = 0; i < nunits; ++i) tu[i] = s->units[s->uorder[i]];
+        size_t nbig = 0, k = 0;
+        int blk[2] = {64, 64}, lds[2] = {0, 0};
+        for (int pass = 0; pass < 2; ++pass)
+            for (size_t i = 0; i < nunits; ++i) {
+                const fafd_sunit *un = &s->units[s->uorder[i]];
+                const int big = un->pad > SV_BLOCK_SMALL;
+                if (big != (pass == 0)) continue;
+                const int th = (un->pad + 63) & ~63;
+                const int lb = s->ulds[s->uorder[i]];
+                if (th > blk[!big]) blk[!big] = th;
+                if (lb > lds[!big]) lds[!big] = lb;
+                tu[k] = *un;
+                tu[k++].pad = 0;
+                if (big) ++nbig;
+            }
         memcpy(ts, s->sigs, nsig * sizeof(fafd_ssig));
-        if (fafk_serve(s->splans_dev, tu, (int)nunits, ts, NULL, 0, block, (size_t)lds, st)) {
+        if ((nbig && fafk_serve(s->splans_dev, tu, (int)nbig, ts, NULL, 0, blk[0],
+                                (size_t)lds[0], st)) ||
+            (nunits > nbig && fafk_serve(s->splans_dev, tu + nbig, (int)(nunits - nbig), ts,
+                                         NULL, 0, blk[1], (size_t)lds[1], st))) {
             set_err(s, fafk_last_error());
             return -1;
         }
@@ -709,6 +732,9 @@ static int launch(faf_server *s, slot *sl, int take) {
     if (fafk_event_record(sl->ev_done, st)) return -1;
     atomic_fetch_add(&s->launches, 1);
     atomic_fetch_add(&s->launched_reqs, (uint64_t)take);
+    atomic_fetch_add(&s->long_reqs, (uint64_t)nl);
+    sl->t_launched = now_ns();
+    atomic_fetch_add(&s->host_ns, sl->t_launched - t0);
     sl->busy = 1;
     return 0;
 }
@@ -730,6 +756,7 @@ static int poll_slots(faf_server *s) {
         if (q == 0) continue;
         int from = sl->small_done ? sl->nsmall : 0;
         if (q < 0) set_err(s, fafk_last_error());
+        atomic_fetch_add(&s->device_ns, now_ns() - sl->t_launched);
         finish(s, sl->reqs + from, sl->nreq - from, q == 1 ? ST_DONE : ST_FAILED);
         sl->busy = 0;
         did = 1;
@@ -772,6 +799,7 @@ static int maybe_launch(faf_server *s) {
         /* plain per-call service: wait for this request before the next */
         rc = fafk_event_sync(free_sl->ev_done);
         if (rc) set_err(s, fafk_last_error());
+        atomic_fetch_add(&s->device_ns, now_ns() - free_sl->t_launched);
         finish(s, free_sl->reqs, free_sl->nreq, rc ? ST_FAILED : ST_DONE);
         free_sl->busy = 0;
     }
@@ -865,7 +893,7 @@ void faf_server_destroy(faf_server *s) {
     free(s->fr);
     free(s->inbox.v); free(s->rel.v); free(s->pend.v); free(s->ready.v);
     free(s->bk_e); free(s->bk_cnt); free(s->bk_off);
-    free(s->sigs); free(s->units); free(s->ucost); free(s->uorder);
+    free(s->sigs); free(s->units); free(s->ucost); free(s->uorder); free(s->ulds);
     pthread_mutex_destroy(&s->plan_mu);
     pthread_mutex_destroy(&s->in_mu);
     pthread_mutex_destroy(&s->done_mu);
@@ -1092,6 +1120,12 @@ int faf_server_stats_get(faf_server *s, faf_server_stats *st) {
     uint64_t lr = atomic_load(&s->launched_reqs);
     st->mean_batch = st->launches ? (double)lr / (double)st->launches : 0.0;
     st->admission_waits = atomic_load(&s->admission_waits);
+    st->long_requests = atomic_load(&s->long_reqs);
+    if (st->launches) {
+        st->host_us_per_batch = (double)atomic_load(&s->host_ns) / 1e3 / (double)st->launches;
+        st->device_us_per_batch =
+            (double)atomic_load(&s->device_ns) / 1e3 / (double)st->launches;
+    }
     if (m) {
         qsort(lat, m, sizeof(uint64_t), cmp_u64);
         st->p50_us = (double)lat[(m - 1) / 2] / 1e3;
@@ -1117,6 +1151,9 @@ void faf_server_stats_reset(faf_server *s) {
     pthread_mutex_unlock(&s->done_mu);
     atomic_store(&s->launches, 0);
     atomic_store(&s->launched_reqs, 0);
+    atomic_store(&s->long_reqs, 0);
+    atomic_store(&s->host_ns, 0);
+    atomic_store(&s->device_ns, 0);
     atomic_store(&s->admission_waits, 0);
 }
 

@@ -139,6 +139,9 @@ typedef struct faf_server_stats {
     double gb_per_s;             /* algorithmic bytes (input + result) / span */
     size_t peak_hbm_bytes;       /* arena bytes held at the peak */
     uint64_t admission_waits;    /* requests that waited for budget */
+    uint64_t long_requests;      /* requests run through their own plan launches */
+    double host_us_per_batch;    /* worker time to build and launch a batch */
+    double device_us_per_batch;  /* launch -> completion seen by the worker */
 } faf_server_stats;
 
 faf_server_config faf_server_config_init(void);
