@@ -84,8 +84,17 @@ enum {
     FAFD_SK_BLUE = 2,
     FAFD_SK_R2C = 3,      /* real in, interleaved Hermitian out */
     FAFD_SK_STFT = 4,     /* one windowed frame -> interleaved Hermitian */
-    FAFD_SK_CWT = 5       /* one segment: rfft then analytic rows [r0, r1) */
+    FAFD_SK_CWT = 5,      /* one segment: rfft then analytic rows [r0, r1) */
+    /* phases of lengths beyond one workgroup (four-step n = n1 n2), one
+     * signal-table entry per unit: in, out, avail = CWT filter row, r0 =
+     * first column / row / bin of the unit */
+    FAFD_SK_FSCOL = 6,    /* fs_per columns: gather stride n2, FFT n1, twiddle, ws[k1][n2] */
+    FAFD_SK_FSROW = 7,    /* fs_per rows of ws: FFT n2, write out[k1 + n1 k2] */
+    FAFD_SK_POST = 8      /* Kiss post-pass over fs_per bins of a packed spectrum */
 };
+
+/* what an FSCOL unit gathers */
+enum { FAFD_FSIN_INTER = 0, FAFD_FSIN_PAIR = 1, FAFD_FSIN_SPREAD = 2 };
 
 typedef struct fafd_splan {
     int32_t kind;
@@ -102,6 +111,14 @@ typedef struct fafd_splan {
     int64_t psi_stride;
     int32_t rows;        /* CWT rows */
     int32_t lds_per;     /* LDS float2 slots one signal needs */
+    /* four-step phases */
+    int32_t inv;         /* inverse transforms in this phase */
+    int32_t fs_other;    /* FSCOL: n2; FSROW: n1; POST: packed length nc */
+    int32_t fs_per;      /* columns / rows / bins per unit */
+    int32_t fs_in;       /* FAFD_FSIN_* */
+    int32_t fs_nc;       /* SPREAD: one-sided spectrum holds bins 0..fs_nc */
+    float scale;         /* FSROW output scale */
+    const void *twn;     /* FSCOL: W_n^m, n = n1 n2, in this phase's direction */
 } fafd_splan;
 
 typedef struct fafd_sunit {

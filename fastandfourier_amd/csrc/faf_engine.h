@@ -1046,6 +1046,92 @@ FAF_DEV void post_pass_inter(const Lds &z, const float2 *post, int nc, int lt, i
     }
 }
 
+/* One unit of a four-step phase (lengths beyond one workgroup).  All
+ * threads of the workgroup take part; FSCOL and FSROW hold fs_per FFTs of
+ * pl->a side by side in LDS. */
+FAF_DEV void serve_phase_unit(const fafd_splan *pl, const fafd_ssig &sg, int tid, int bd) {
+    if (pl->kind == FAFD_SK_POST) {
+        const int nc = pl->fs_other;
+        const int k1 = sg.r0 + pl->fs_per < nc / 2 + 1 ? sg.r0 + pl->fs_per : nc / 2 + 1;
+        const float2 *z = (const float2 *)sg.in;
+        float2 *out = (float2 *)sg.out;
+        for (int k = sg.r0 + tid; k < k1; k += bd) {
+            const int m = nc - k;
+            float2 xk, xm;
+            if (k == 0) {
+                const float2 v = z[0];
+                xk = make_float2(v.x + v.y, 0.0f);
+                xm = make_float2(v.x - v.y, 0.0f);
+            } else {
+                kiss_post(z[k], z[m], ((const float2 *)pl->post)[k], xk, xm);
+            }
+            out[k] = xk;
+            out[m] = xm;
+        }
+        return;
+    }
+    const fafd_fft &P = pl->a;
+    const int n = P.n, per = pl->fs_per, T = P.threads, other = pl->fs_other;
+    const int sid = tid / T, lt = tid - sid * T;
+    if (pl->kind == FAFD_SK_FSCOL) {
+        /* columns c0 .. c0+per of x viewed as [n1][n2]: element (k1, c) */
+        const int n2 = other, c0 = sg.r0;
+        const float2 *xc = (const float2 *)sg.in;
+        const float *psi = pl->psi + sg.avail * pl->psi_stride;
+        for (int f = tid; f < per * n; f += bd) {
+            const int c = f % per, r = f / per;
+            const long i = (long)r * n2 + c0 + c;
+            float2 v = make_float2(0.0f, 0.0f);
+            if (c0 + c < n2) {
+                if (pl->fs_in == FAFD_FSIN_INTER) {
+                    v = xc[i];
+                } else if (pl->fs_in == FAFD_FSIN_PAIR) {
+                    v = make_float2(sg.in[2 * i], sg.in[2 * i + 1]);
+                } else if (i <= pl->fs_nc) {   /* analytic row spectrum X psi_r */
+                    float w = psi[i];
+                    if (i != 0 && i != pl->fs_nc) w *= 2.0f;
+                    v = cscale(xc[i], w);
+                }
+            }
+            Lds{faf_smem + c * P.plane}.st(r, v);
+        }
+        __syncthreads();
+        Lds lds{faf_smem + sid * P.plane};
+        if (pl->inv) fft_run<true>(P, lds, lds, lds, lt, sid < per);
+        else fft_run<false>(P, lds, lds, lds, lt, sid < per);
+        __syncthreads();
+        const float2 *tw = (const float2 *)pl->twn;
+        float2 *ws = (float2 *)sg.out;
+        for (int f = tid; f < per * n; f += bd) {
+            const int c = f % per, k1 = f / per;
+            if (c0 + c >= n2) continue;
+            const float2 v = Lds{faf_smem + c * P.plane}.ld(k1);
+            ws[(long)k1 * n2 + c0 + c] = cmul(v, tw[(long)k1 * (c0 + c)]);
+        }
+        return;
+    }
+    /* FSROW: rows k0 .. k0+per of ws [n1][n2], FFT n2, out[k1 + n1 k2] */
+    const int n1 = other, k0 = sg.r0;
+    const float2 *ws = (const float2 *)sg.in;
+    for (int r = 0; r < per && k0 + r < n1; ++r) {
+        Lds row{faf_smem + r * P.plane};
+        const long b = (long)(k0 + r) * n;
+        for (int c = tid; c < n; c += bd) row.st(c, ws[b + c]);
+    }
+    __syncthreads();
+    Lds lds{faf_smem + sid * P.plane};
+    const int act = sid < per && k0 + sid < n1;
+    if (pl->inv) fft_run<true>(P, lds, lds, lds, lt, act);
+    else fft_run<false>(P, lds, lds, lds, lt, act);
+    __syncthreads();
+    float2 *out = (float2 *)sg.out;
+    for (int f = tid; f < per * n; f += bd) {
+        const int r = f % per, k2 = f / per;
+        if (k0 + r >= n1) continue;
+        out[(long)(k0 + r) + (long)n1 * k2] = cscale(Lds{faf_smem + r * P.plane}.ld(k2), pl->scale);
+    }
+}
+
 #ifndef FAF_ENGINE_NO_PLAIN_KERNELS
 template <int MT>
 __global__ __launch_bounds__(MT) void k_serve(const fafd_splan *plans, const fafd_sunit *units, int nunits,
@@ -1063,6 +1149,11 @@ __global__ __launch_bounds__(MT) void k_serve(const fafd_splan *plans, const faf
         if (u >= nunits) break;
         const fafd_sunit un = units[u];
         const fafd_splan *pl = plans + un.plan;
+        if (pl->kind >= FAFD_SK_FSCOL) {
+            serve_phase_unit(pl, sigs[un.first], tid, bd);
+            __syncthreads();
+            continue;
+        }
         const int T = pl->threads;
         const int sid = tid / T, lt = tid - sid * T;
         const int active = sid < un.count && (sid + 1) * T <= bd;

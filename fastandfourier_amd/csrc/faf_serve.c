@@ -44,11 +44,36 @@
 
 enum { ST_QUEUED = 0, ST_ADMITTED, ST_LAUNCHED, ST_DONE, ST_FAILED };
 
+/* buffers a phase reads or writes: the request's input, its result, or a
+ * region of its workspace (which follows the result in its arena block) */
+enum { SV_IN = -1, SV_OUT = -2 };
+enum { SV_WS0 = 0, SV_WS1, SV_WSZ, SV_WSR, SV_NREG };
+#define SV_MAX_PHASES 5
+
+/* one phase of a request longer than one workgroup holds: `units` units
+ * of the device plan `splan` (repeated for every CWT filter row when
+ * each_row is set) */
+typedef struct sphase {
+    int splan;
+    int units;               /* units per request (per row when each_row) */
+    int per;                 /* columns / rows / bins per unit */
+    int T;                   /* threads of one unit */
+    int lds;                 /* LDS bytes of one unit */
+    int each_row;
+    int src, dst;            /* SV_IN, SV_OUT or a workspace region */
+    int src_row, dst_row;    /* advance src / dst by one row per filter row */
+    double cost;             /* relative work of one unit */
+} sphase;
+
 typedef struct sentry {
     int kind;                /* FAF_REQ_* */
     size_t n;                /* length (STFT: window) */
-    int small;               /* runs inside the serving kernel */
-    int splan;               /* index in the device plan table */
+    int small;               /* one plan inside the serving kernel */
+    int splan;               /* index in the device plan table (small) */
+    int nph;                 /* phases of a long request */
+    sphase ph[SV_MAX_PHASES];
+    size_t reg[SV_NREG];     /* workspace region offsets */
+    size_t row_bytes;        /* CWT: bytes of one analytic row */
     int T;                   /* threads per signal (CWT: per unit) */
     int lds_per;             /* LDS float2 per signal (CWT: per unit) */
     int spb;                 /* signals per unit */
@@ -132,7 +157,7 @@ struct faf_server {
     fafd_sunit *units;
     double *ucost;
     size_t units_cap;
-    int *uorder, *ulds;
+    int *uorder, *ulds, *uthr, *uphase;
     /* stats */
     int64_t stats_first;
     _Atomic uint64_t launches, launched_reqs, admission_waits, long_reqs;
@@ -265,17 +290,126 @@ static int upload_splan(faf_server *s, const fafd_splan *sp) {
 }
 
 static int shape_fits(sentry *e) {
-    if (e->T < 1 || e->T > SV_BLOCK_MAX) return 0;
+    if (e->T < 1 || e->T > SV_BLOCK_SMALL) return 0;
     /* the serving kernel keeps a few bytes of static LDS beside the table */
     const size_t cap = FAFD_MAX_LDS_BYTES - SV_ALIGN;
     size_t per = (size_t)e->lds_per * 8;
     if (per > cap) return 0;
-    /* units stay within 512 threads (the serving kernel's register budget
-     * without spills) unless one signal alone needs more */
-    int by_t = e->T <= SV_BLOCK_SMALL ? SV_BLOCK_SMALL / e->T : 1;
+    /* units stay within 512 threads, the serving kernel's register budget
+     * without spills */
+    int by_t = SV_BLOCK_SMALL / e->T;
     int by_l = (int)(cap / per);
     e->spb = by_t < by_l ? by_t : by_l;
     return e->spb >= 1;
+}
+
+/* FFTs of P side by side in one unit: at most 16, within 512 threads and
+ * the LDS */
+static int fs_per_unit(const fafd_fft *P) {
+    int per = 16;
+    while (per > 1 && (per * P->threads > SV_BLOCK_SMALL ||
+                       (size_t)per * (size_t)P->plane * 8 > FAFD_MAX_LDS_BYTES - SV_ALIGN))
+        per >>= 1;
+    return per;
+}
+
+static int add_phase(faf_server *s, sentry *e, const fafd_splan *sp, int units, int per, int T,
+                     int lds, int src, int dst, double cost) {
+    if (e->nph >= SV_MAX_PHASES || T > SV_BLOCK_SMALL) {
+        fafi_err("serve: phase shape out of range");
+        return -1;
+    }
+    int idx = upload_splan(s, sp);
+    if (idx < 0) return -1;
+    sphase *ph = &e->ph[e->nph++];
+    memset(ph, 0, sizeof *ph);
+    ph->splan = idx;
+    ph->units = units;
+    ph->per = per;
+    ph->T = T;
+    ph->lds = lds;
+    ph->src = src;
+    ph->dst = dst;
+    ph->cost = cost;
+    return 0;
+}
+
+/* four-step phases of a length-n C2C (direction inv) from src to dst;
+ * fsin selects how the column pass gathers its input */
+static int add_4step(faf_server *s, sentry *e, size_t n, int inv, int fsin, size_t nc_spread,
+                     const float *psi, long psi_stride, float scale, int src, int mid, int dst) {
+    int n1, n2;
+    if (fafi_split_4step(n, &n1, &n2)) {
+        fafi_err("serve: length %zu has no four-step split", n);
+        return -1;
+    }
+    fafd_splan col, row;
+    memset(&col, 0, sizeof col);
+    memset(&row, 0, sizeof row);
+    if (fafi_make_fft((size_t)n1, inv, 0, NULL, 0, &col.a) ||
+        fafi_make_fft((size_t)n2, inv, 0, NULL, 0, &row.a))
+        return -1;
+    col.kind = FAFD_SK_FSCOL;
+    col.inv = inv;
+    col.fs_other = n2;
+    col.fs_per = fs_per_unit(&col.a);
+    col.fs_in = fsin;
+    col.fs_nc = (int32_t)nc_spread;
+    col.psi = psi;
+    col.psi_stride = psi_stride;
+    col.twn = fafi_twiddles(n, inv);
+    row.kind = FAFD_SK_FSROW;
+    row.inv = inv;
+    row.fs_other = n1;
+    row.fs_per = fs_per_unit(&row.a);
+    row.scale = scale;
+    if (!col.twn) return -1;
+    const int cu = (n2 + col.fs_per - 1) / col.fs_per, ru = (n1 + row.fs_per - 1) / row.fs_per;
+    if (add_phase(s, e, &col, cu, col.fs_per, col.fs_per * col.a.threads,
+                  col.fs_per * col.a.plane * 8, src, mid, col.fs_per * nlogn(n1)) ||
+        add_phase(s, e, &row, ru, row.fs_per, row.fs_per * row.a.threads,
+                  row.fs_per * row.a.plane * 8, mid, dst, row.fs_per * nlogn(n2)))
+        return -1;
+    return 0;
+}
+
+/* Kiss post-pass over a packed length-nc spectrum, src -> dst */
+static int add_post(faf_server *s, sentry *e, size_t nc, const void *post, int src, int dst) {
+    fafd_splan sp;
+    memset(&sp, 0, sizeof sp);
+    sp.kind = FAFD_SK_POST;
+    sp.fs_other = (int32_t)nc;
+    sp.fs_per = 2048;
+    sp.post = post;
+    const int units = (int)((nc / 2 + 1 + 2047) / 2048);
+    return add_phase(s, e, &sp, units, 2048, 256, 0, src, dst, 2048.0);
+}
+
+/* packed rfft of length 2 nc from src into the Hermitian spectrum at dst:
+ * one serving-kernel signal when nc fits a 512-thread workgroup, else
+ * four-step phases plus the post-pass */
+static int add_rfft(faf_server *s, sentry *e, fafh_plan *q, size_t nc, int src, int dst) {
+    fafd_fft a;
+    if (fafi_radix_fits(nc, 0) && !fafi_make_fft(nc, 0, 0, NULL, 0, &a) &&
+        a.threads <= SV_BLOCK_SMALL) {
+        fafd_splan sp;
+        memset(&sp, 0, sizeof sp);
+        sp.kind = FAFD_SK_R2C;
+        sp.a = a;
+        sp.post = fafi_kiss_table(nc, 0);
+        sp.len = (int32_t)(2 * nc);
+        sp.threads = a.threads;
+        sp.lds_per = a.plane;
+        if (!sp.post) return -1;
+        return add_phase(s, e, &sp, 1, 1, a.threads, a.plane * 8, src, dst, nlogn(nc));
+    }
+    (void)q;
+    const void *post = fafi_kiss_table(nc, 0);
+    if (!post) return -1;
+    return add_4step(s, e, nc, 0, FAFD_FSIN_PAIR, 0, NULL, 0, 1.0f, src, SV_WS0, SV_WS1) ||
+                   add_post(s, e, nc, post, SV_WS1, dst)
+               ? -1
+               : 0;
 }
 
 static int build_entry(faf_server *s, sentry *e) {
@@ -285,58 +419,63 @@ static int build_entry(faf_server *s, sentry *e) {
     fafd_splan sp;
     memset(&sp, 0, sizeof sp);
     size_t n = e->n;
-    int rc = 0;
     switch (e->kind) {
     case FAF_REQ_C2C:
-        rc = fafi_plan_c2c(q, n, 0, 0);
-        if (rc) return -1;
-        q->interleaved = 1;
+        if (fafi_plan_c2c(q, n, 0, 0)) return -1;
         e->cost = nlogn((double)n);
-        if (q->kind == PK_C2C) {
+        if (q->kind == PK_C2C && q->a.threads <= SV_BLOCK_SMALL) {
             sp.kind = FAFD_SK_C2C;
             sp.a = q->a;
             e->T = q->a.threads;
             e->lds_per = q->a.plane;
-        } else if (q->kind == PK_RADER) {
+        } else if (q->kind == PK_RADER && q->a.threads <= SV_BLOCK_SMALL) {
             sp.kind = FAFD_SK_RADER;
             sp.a = q->a;
             sp.b = q->b;
             sp.rader = q->rader;
             e->T = q->a.threads;
             e->lds_per = q->a.plane + (int)(n + (n >> 4) + 1);
-        } else if (q->kind == PK_BLUE) {
+        } else if (q->kind == PK_BLUE && q->a.threads <= SV_BLOCK_SMALL) {
             sp.kind = FAFD_SK_BLUE;
             sp.a = q->a;
             sp.b = q->b;
             sp.blue = q->blue;
             e->T = q->a.threads;
             e->lds_per = q->a.plane;
+        } else if (q->kind == PK_C2C || q->kind == PK_C2C_4S) {
+            e->reg[SV_WS0] = 0;
+            if (add_4step(s, e, n, 0, FAFD_FSIN_INTER, 0, NULL, 0, 1.0f, SV_IN, SV_WS0, SV_OUT))
+                return -1;
+            e->ws_bytes = al(n * 8);
+            return 0;
         } else {
-            e->small = 0;
-            e->ws_bytes = fafi_run_ws_bytes(q, 1);
-            return 0;
+            fafi_err("serve: C2C length %zu is beyond the serving kernel", n);
+            return -1;
         }
         break;
-    case FAF_REQ_RFFT:
+    case FAF_REQ_RFFT: {
         if (n < 4 || (n & 1)) { fafi_err("serve: rfft length %zu must be even and >= 4", n); return -1; }
-        rc = fafi_plan_real(q, n, 0);
-        if (rc) return -1;
-        q->interleaved = 1;
-        e->cost = nlogn((double)n / 2);
-        if (q->kind != PK_R2C) {
-            e->small = 0;
-            e->ws_bytes = fafi_run_ws_bytes(q, 1);
-            return 0;
+        const size_t nc = n / 2;
+        e->cost = nlogn((double)nc);
+        fafd_fft a;
+        if (fafi_radix_fits(nc, 0) && !fafi_make_fft(nc, 0, 0, NULL, 0, &a) &&
+            a.threads <= SV_BLOCK_SMALL) {
+            sp.kind = FAFD_SK_R2C;
+            sp.a = a;
+            sp.post = fafi_kiss_table(nc, 0);
+            if (!sp.post) return -1;
+            e->T = a.threads;
+            e->lds_per = a.plane;
+            break;
         }
-        sp.kind = FAFD_SK_R2C;
-        sp.a = q->a;
-        sp.post = q->post;
-        e->T = q->a.threads;
-        e->lds_per = q->a.plane;
-        break;
+        e->reg[SV_WS0] = 0;
+        e->reg[SV_WS1] = al(nc * 8);
+        if (add_rfft(s, e, q, nc, SV_IN, SV_OUT)) return -1;
+        e->ws_bytes = 2 * al(nc * 8);
+        return 0;
+    }
     case FAF_REQ_STFT:
-        rc = fafi_plan_stft(q, n, s->cfg.stft_hop);
-        if (rc) return -1;
+        if (fafi_plan_stft(q, n, s->cfg.stft_hop)) return -1;
         e->hop = s->cfg.stft_hop;
         e->cost = nlogn((double)n / 2);
         sp.kind = FAFD_SK_STFT;
@@ -356,21 +495,39 @@ static int build_entry(faf_server *s, sentry *e) {
         if (fafi_cwt_build(q, &c, -1)) return -1;
         e->rows = q->rows_total;
         e->cost = nlogn((double)n) * SV_CWT_ROWS_PER_UNIT;
-        if (q->sub_r2c) {
-            e->small = 0;
-            e->ws_bytes = fafi_cwt_ws_bytes(q, 1, 1);
-            return 0;
+        const int fits = !q->sub_r2c && q->a.threads <= SV_BLOCK_SMALL &&
+                         q->full_inv.threads <= SV_BLOCK_SMALL;
+        if (fits) {
+            sp.kind = FAFD_SK_CWT;
+            sp.a = q->a;
+            sp.b = q->full_inv;
+            sp.post = q->post;
+            sp.psi = q->psi_dev;
+            sp.psi_stride = q->psi_stride;
+            sp.rows = q->rows_total;
+            e->T = q->a.threads > q->full_inv.threads ? q->a.threads : q->full_inv.threads;
+            e->lds_per = q->a.plane + q->full_inv.plane;
+            break;
         }
-        sp.kind = FAFD_SK_CWT;
-        sp.a = q->a;
-        sp.b = q->full_inv;
-        sp.post = q->post;
-        sp.psi = q->psi_dev;
-        sp.psi_stride = q->psi_stride;
-        sp.rows = q->rows_total;
-        e->T = q->a.threads > q->full_inv.threads ? q->a.threads : q->full_inv.threads;
-        e->lds_per = q->a.plane + q->full_inv.plane;
-        break;
+        /* rfft into Z, then per filter row an inverse four-step over the
+         * analytic spectrum Z psi_r gathered on the fly */
+        const size_t nc = n / 2;
+        e->reg[SV_WS0] = 0;
+        e->reg[SV_WS1] = al(nc * 8);
+        e->reg[SV_WSZ] = 2 * al(nc * 8);
+        e->reg[SV_WSR] = e->reg[SV_WSZ] + al((nc + 1) * 8);
+        e->row_bytes = n * 8;
+        e->ws_bytes = e->reg[SV_WSR] + al((size_t)e->rows * n * 8);
+        if (add_rfft(s, e, q, nc, SV_IN, SV_WSZ)) return -1;
+        const int first = e->nph;
+        if (add_4step(s, e, n, 1, FAFD_FSIN_SPREAD, nc, q->psi_dev, q->psi_stride,
+                      (float)(1.0 / (double)n), SV_WSZ, SV_WSR, SV_OUT))
+            return -1;
+        for (int i = first; i < e->nph; ++i) e->ph[i].each_row = 1;
+        e->ph[first].dst_row = 1;                     /* column pass: Z -> WSR row */
+        e->ph[first + 1].src_row = 1;                 /* row pass: WSR row -> out row */
+        e->ph[first + 1].dst_row = 1;
+        return 0;
     }
     default:
         fafi_err("serve: unknown request kind %d", e->kind);
@@ -544,34 +701,21 @@ static int by_cost_desc(const void *a, const void *b) {
     return ca < cb ? 1 : (ca > cb ? -1 : 0);
 }
 
-/* run the long (non-kernel-table) request r on stream st */
-static int run_long(faf_server *s, srec *r, void *st) {
-    fafh_plan *q = r->e->plan;
-    q->ws_ext = r->out + al(r->out_bytes);
-    q->ws_ext_bytes = r->blk_bytes - al(r->out_bytes);
-    if (!q->ws_ext_bytes) q->ws_ext = NULL;
-    int rc;
-    float *o = (float *)r->out;
-    size_t n = r->n;
-    switch (r->kind) {
-    case FAF_REQ_C2C:
-        rc = fafi_run_c2c(q, r->in, NULL, (long)n, o, NULL, (long)n, 1, st);
-        break;
-    case FAF_REQ_RFFT:
-        rc = fafi_run_r2c(q, r->in, (long)n, o, NULL, (long)(n / 2 + 1), 1, st);
-        break;
-    default:
-        rc = fafi_run_cwt(q, r->in, (long)n, o, (long)r->e->rows * (long)n, 1, 1, st);
-        break;
-    }
-    q->ws_ext = NULL;
-    q->ws_ext_bytes = 0;
-    return rc;
+/* where buffer `b` of request r starts, for filter row `row` */
+static const float *buf_of(const srec *r, int b, int row, int advance) {
+    const char *p;
+    if (b == SV_IN) p = (const char *)r->in;
+    else if (b == SV_OUT) p = r->out;
+    else p = r->out + al(r->out_bytes) + r->e->reg[b];
+    if (advance) p += (size_t)row * r->e->row_bytes;
+    return (const float *)p;
 }
 
 /* Pops `take` admitted requests into the slot (short ones first) and
- * launches them.  Returns 0, -1 when the popped requests must fail (after
- * the slot's stream has drained), or -2 when nothing was popped. */
+ * launches them: one serving-kernel launch per phase, the short requests
+ * and the first phase of the long ones sharing the first launch.  Returns
+ * 0, -1 when the popped requests must fail (after the slot's stream has
+ * drained), or -2 when nothing was popped. */
 static int launch(faf_server *s, slot *sl, int take) {
     const uint64_t stamp = ++s->stamp;
     const uint64_t t0 = now_ns();
@@ -581,8 +725,8 @@ static int launch(faf_server *s, slot *sl, int take) {
         sl->reqs = v;
         sl->cap = take;
     }
-    int ns = 0, nl = 0, nb = 0;
-    size_t nsig = 0;
+    int ns = 0, nl = 0, nb = 0, nph = 1;
+    size_t nsig = 0, nunits = 0;
     for (int i = 0; i < take; ++i)
         if (rec(s, ring_at(&s->ready, (size_t)i))->e->small) ++ns;
     for (int i = 0; i < take; ++i) {
@@ -620,114 +764,149 @@ static int launch(faf_server *s, slot *sl, int take) {
         s->bk_cnt[e->bucket] += k;
         nsig += (size_t)k;
     }
+    const size_t nsig_small = nsig;
+    for (int b = 0, off = 0; b < nb; ++b) {
+        s->bk_off[b] = off;
+        off += s->bk_cnt[b];
+        nunits += (size_t)((s->bk_cnt[b] + s->bk_e[b]->spb - 1) / s->bk_e[b]->spb);
+    }
+    for (int i = ns; i < take; ++i) {
+        const sentry *e = rec(s, sl->reqs[i])->e;
+        for (int p = 0; p < e->nph; ++p)
+            nunits += (size_t)e->ph[p].units * (size_t)(e->ph[p].each_row ? e->rows : 1);
+        if (e->nph > nph) nph = e->nph;
+    }
+    for (int i = ns; i < take; ++i) {
+        const sentry *e = rec(s, sl->reqs[i])->e;
+        for (int p = 0; p < e->nph; ++p)
+            nsig += (size_t)e->ph[p].units * (size_t)(e->ph[p].each_row ? e->rows : 1);
+    }
+    if (grow((void **)&s->sigs, &s->sigs_cap, nsig, sizeof(fafd_ssig))) return -1;
+    if (grow((void **)&s->units, &s->units_cap, nunits, sizeof(fafd_sunit))) return -1;
+    s->ucost = realloc(s->ucost, s->units_cap * sizeof(double));
+    s->uorder = realloc(s->uorder, s->units_cap * sizeof(int));
+    s->ulds = realloc(s->ulds, s->units_cap * sizeof(int));
+    s->uthr = realloc(s->uthr, s->units_cap * sizeof(int));
+    s->uphase = realloc(s->uphase, s->units_cap * sizeof(int));
+    if (!s->ucost || !s->uorder || !s->ulds || !s->uthr || !s->uphase) return -1;
+
+    /* short requests: signals grouped by plan, units of at most spb */
+    for (int i = 0; i < ns; ++i) {
+        srec *r = rec(s, sl->reqs[i]);
+        sentry *e = r->e;
+        fafd_ssig *g = s->sigs + s->bk_off[e->bucket];
+        if (r->kind == FAF_REQ_STFT) {
+            const size_t fr = stft_frames(s, r->n), bins = e->n / 2 + 1;
+            for (size_t f = 0; f < fr; ++f) {
+                g[f].in = r->in + f * e->hop;
+                g[f].out = (float *)r->out + 2 * f * bins;
+                g[f].avail = (int64_t)(r->n - f * e->hop);
+                g[f].r0 = g[f].r1 = 0;
+            }
+            s->bk_off[e->bucket] += (int)fr;
+        } else if (r->kind == FAF_REQ_CWT) {
+            int k = 0;
+            for (int r0 = 0; r0 < e->rows; r0 += SV_CWT_ROWS_PER_UNIT, ++k) {
+                g[k].in = r->in;
+                g[k].out = (float *)r->out;
+                g[k].avail = (int64_t)r->n;
+                g[k].r0 = r0;
+                g[k].r1 = r0 + SV_CWT_ROWS_PER_UNIT < e->rows ? r0 + SV_CWT_ROWS_PER_UNIT
+                                                               : e->rows;
+            }
+            s->bk_off[e->bucket] += k;
+        } else {
+            g[0].in = r->in;
+            g[0].out = (float *)r->out;
+            g[0].avail = (int64_t)r->n;
+            g[0].r0 = g[0].r1 = 0;
+            s->bk_off[e->bucket] += 1;
+        }
+    }
+    size_t u = 0;
+    for (int b = 0, first = 0; b < nb; ++b) {
+        sentry *e = s->bk_e[b];
+        for (int i = 0; i < s->bk_cnt[b]; i += e->spb, ++u) {
+            int c = s->bk_cnt[b] - i < e->spb ? s->bk_cnt[b] - i : e->spb;
+            s->units[u].plan = e->splan;
+            s->units[u].count = c;
+            s->units[u].first = first + i;
+            s->units[u].pad = 0;
+            s->ucost[u] = e->cost * (double)c;
+            s->ulds[u] = c * e->lds_per * 8;
+            s->uthr[u] = c * e->T;
+            s->uphase[u] = 0;
+        }
+        first += s->bk_cnt[b];
+    }
+    /* long requests: one unit and one signal per column / row / bin block */
+    size_t g = nsig_small;
+    for (int i = ns; i < take; ++i) {
+        const srec *r = rec(s, sl->reqs[i]);
+        const sentry *e = r->e;
+        for (int p = 0; p < e->nph; ++p) {
+            const sphase *ph = &e->ph[p];
+            const int reps = ph->each_row ? e->rows : 1;
+            for (int row = 0; row < reps; ++row)
+                for (int k = 0; k < ph->units; ++k, ++g, ++u) {
+                    fafd_ssig *sg = &s->sigs[g];
+                    sg->in = buf_of(r, ph->src, row, ph->src_row);
+                    sg->out = (float *)buf_of(r, ph->dst, row, ph->dst_row);
+                    sg->avail = row;
+                    sg->r0 = k * ph->per;
+                    sg->r1 = 0;
+                    s->units[u].plan = ph->splan;
+                    s->units[u].count = 1;
+                    s->units[u].first = (int32_t)g;
+                    s->units[u].pad = 0;
+                    s->ucost[u] = ph->cost;
+                    s->ulds[u] = ph->lds;
+                    s->uthr[u] = ph->T;
+                    s->uphase[u] = p;
+                }
+        }
+    }
+    /* per phase, most expensive units first */
+    int pcount[SV_MAX_PHASES + 1] = {0}, pstart[SV_MAX_PHASES + 1] = {0};
+    for (size_t k = 0; k < nunits; ++k) pcount[s->uphase[k]]++;
+    for (int p = 1; p <= nph; ++p) pstart[p] = pstart[p - 1] + pcount[p - 1];
+    {
+        int fill[SV_MAX_PHASES];
+        for (int p = 0; p < nph; ++p) fill[p] = pstart[p];
+        for (size_t k = 0; k < nunits; ++k) s->uorder[fill[s->uphase[k]]++] = (int)k;
+    }
+    g_sort_srv = s;
+    for (int p = 0; p < nph; ++p)
+        qsort(s->uorder + pstart[p], (size_t)pcount[p], sizeof(int), by_cost_desc);
+    size_t need = nunits * sizeof(fafd_sunit) + nsig * sizeof(fafd_ssig);
+    if (need > sl->tab_bytes) {
+        fafk_host_free(sl->tab);
+        size_t nb2 = sl->tab_bytes ? sl->tab_bytes : 65536;
+        while (nb2 < need) nb2 *= 2;
+        sl->tab = fafk_host_alloc(nb2);
+        sl->tab_bytes = sl->tab ? nb2 : 0;
+        if (!sl->tab) { set_err(s, fafk_last_error()); return -1; }
+    }
+    fafd_sunit *tu = sl->tab;
+    fafd_ssig *ts = (fafd_ssig *)(tu + nunits);
+    for (size_t k = 0; k < nunits; ++k) tu[k] = s->units[s->uorder[k]];
+    memcpy(ts, s->sigs, nsig * sizeof(fafd_ssig));
     void *st = sl->stream;
-    if (ns) {
-        /* signals grouped by plan, then units of at most spb signals */
-        if (grow((void **)&s->sigs, &s->sigs_cap, nsig, sizeof(fafd_ssig))) return -1;
-        size_t nunits = 0;
-        for (int b = 0, off = 0; b < nb; ++b) {
-            s->bk_off[b] = off;
-            off += s->bk_cnt[b];
-            nunits += (size_t)((s->bk_cnt[b] + s->bk_e[b]->spb - 1) / s->bk_e[b]->spb);
+    for (int p = 0; p < nph; ++p) {
+        if (!pcount[p]) continue;
+        int blk = 64, lds = 0;
+        for (int k = pstart[p]; k < pstart[p] + pcount[p]; ++k) {
+            const int q = s->uorder[k];
+            const int th = (s->uthr[q] + 63) & ~63;
+            if (th > blk) blk = th;
+            if (s->ulds[q] > lds) lds = s->ulds[q];
         }
-        for (int i = 0; i < ns; ++i) {
-            srec *r = rec(s, sl->reqs[i]);
-            sentry *e = r->e;
-            fafd_ssig *g = s->sigs + s->bk_off[e->bucket];
-            if (r->kind == FAF_REQ_STFT) {
-                const size_t fr = stft_frames(s, r->n), bins = e->n / 2 + 1;
-                for (size_t f = 0; f < fr; ++f) {
-                    g[f].in = r->in + f * e->hop;
-                    g[f].out = (float *)r->out + 2 * f * bins;
-                    g[f].avail = (int64_t)(r->n - f * e->hop);
-                    g[f].r0 = g[f].r1 = 0;
-                }
-                s->bk_off[e->bucket] += (int)fr;
-            } else if (r->kind == FAF_REQ_CWT) {
-                int k = 0;
-                for (int r0 = 0; r0 < e->rows; r0 += SV_CWT_ROWS_PER_UNIT, ++k) {
-                    g[k].in = r->in;
-                    g[k].out = (float *)r->out;
-                    g[k].avail = (int64_t)r->n;
-                    g[k].r0 = r0;
-                    g[k].r1 = r0 + SV_CWT_ROWS_PER_UNIT < e->rows ? r0 + SV_CWT_ROWS_PER_UNIT
-                                                                   : e->rows;
-                }
-                s->bk_off[e->bucket] += k;
-            } else {
-                g[0].in = r->in;
-                g[0].out = (float *)r->out;
-                g[0].avail = (int64_t)r->n;
-                g[0].r0 = g[0].r1 = 0;
-                s->bk_off[e->bucket] += 1;
-            }
-        }
-        if (grow((void **)&s->units, &s->units_cap, nunits, sizeof(fafd_sunit))) return -1;
-        s->ucost = realloc(s->ucost, s->units_cap * sizeof(double));
-        s->uorder = realloc(s->uorder, s->units_cap * sizeof(int));
-        s->ulds = realloc(s->ulds, s->units_cap * sizeof(int));
-        if (!s->ucost || !s->uorder || !s->ulds) return -1;
-        size_t u = 0;
-        for (int b = 0, first = 0; b < nb; ++b) {
-            sentry *e = s->bk_e[b];
-            for (int i = 0; i < s->bk_cnt[b]; i += e->spb, ++u) {
-                int c = s->bk_cnt[b] - i < e->spb ? s->bk_cnt[b] - i : e->spb;
-                s->units[u].plan = e->splan;
-                s->units[u].count = c;
-                s->units[u].first = first + i;
-                s->units[u].pad = c * e->T;          /* threads it needs */
-                s->ucost[u] = e->cost * (double)c;
-                s->uorder[u] = (int)u;
-                s->ulds[u] = c * e->lds_per * 8;
-            }
-            first += s->bk_cnt[b];
-        }
-        g_sort_srv = s;
-        qsort(s->uorder, nunits, sizeof(int), by_cost_desc);
-        size_t need = nunits * sizeof(fafd_sunit) + nsig * sizeof(fafd_ssig);
-        if (need > sl->tab_bytes) {
-            fafk_host_free(sl->tab);
-            size_t nb2 = sl->tab_bytes ? sl->tab_bytes : 65536;
-            while (nb2 < need) nb2 *= 2;
-            sl->tab = fafk_host_alloc(nb2);
-            sl->tab_bytes = sl->tab ? nb2 : 0;
-            if (!sl->tab) { set_err(s, fafk_last_error()); return -1; }
-        }
-        /* units wider than 512 threads go first, in a launch of their own
-         * with 1024-thread workgroups; the rest share one 512-wide launch */
-        fafd_sunit *tu = sl->tab;
-        fafd_ssig *ts = (fafd_ssig *)(tu + nunits);
-        size_t nbig = 0, k = 0;
-        int blk[2] = {64, 64}, lds[2] = {0, 0};
-        for (int pass = 0; pass < 2; ++pass)
-            for (size_t i = 0; i < nunits; ++i) {
-                const fafd_sunit *un = &s->units[s->uorder[i]];
-                const int big = un->pad > SV_BLOCK_SMALL;
-                if (big != (pass == 0)) continue;
-                const int th = (un->pad + 63) & ~63;
-                const int lb = s->ulds[s->uorder[i]];
-                if (th > blk[!big]) blk[!big] = th;
-                if (lb > lds[!big]) lds[!big] = lb;
-                tu[k] = *un;
-                tu[k++].pad = 0;
-                if (big) ++nbig;
-            }
-        memcpy(ts, s->sigs, nsig * sizeof(fafd_ssig));
-        if ((nbig && fafk_serve(s->splans_dev, tu, (int)nbig, ts, NULL, 0, blk[0],
-                                (size_t)lds[0], st)) ||
-            (nunits > nbig && fafk_serve(s->splans_dev, tu + nbig, (int)(nunits - nbig), ts,
-                                         NULL, 0, blk[1], (size_t)lds[1], st))) {
+        if (fafk_serve(s->splans_dev, tu + pstart[p], pcount[p], ts, NULL, 0, blk, (size_t)lds,
+                       st)) {
             set_err(s, fafk_last_error());
             return -1;
         }
-        if (nl && fafk_event_record(sl->ev_small, st)) return -1;
-    }
-    for (int i = 0; i < nl; ++i) {
-        srec *r = rec(s, sl->reqs[ns + i]);
-        if (run_long(s, r, st)) {
-            set_err(s, faf_get_error());
-            return -1;
-        }
+        if (p == 0 && ns && nl && fafk_event_record(sl->ev_small, st)) return -1;
     }
     if (fafk_event_record(sl->ev_done, st)) return -1;
     atomic_fetch_add(&s->launches, 1);
@@ -894,6 +1073,7 @@ void faf_server_destroy(faf_server *s) {
     free(s->inbox.v); free(s->rel.v); free(s->pend.v); free(s->ready.v);
     free(s->bk_e); free(s->bk_cnt); free(s->bk_off);
     free(s->sigs); free(s->units); free(s->ucost); free(s->uorder); free(s->ulds);
+    free(s->uthr); free(s->uphase);
     pthread_mutex_destroy(&s->plan_mu);
     pthread_mutex_destroy(&s->in_mu);
     pthread_mutex_destroy(&s->done_mu);
