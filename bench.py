@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--only", default="", help="comma list of extra workloads")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline (for profiler runs of the GPU kernels)")
     return ap.parse_args()
 
 
@@ -115,6 +117,8 @@ def main():
     cpu = None
     log("cpu baseline")
     try:
+        if args.no_cpu:
+            raise RuntimeError("skipped (--no-cpu)")
         import oracle as O
         threads = min(16, len(os.sched_getaffinity(0)))
         xs = x[:8192].cpu().numpy()
@@ -141,6 +145,12 @@ def main():
                "sample": f"{sample} of the {b} N=4096 signals x {reps} pass(es) per timing "
                          f"(same bytes per signal), median of 3 timings of {dt:.2f} s each "
                          f"on {threads} threads, one transform per thread at a time"}
+        # the same restatement on one core, ~3 s of the same signals
+        one = min(sample, max(256, int(3.0 / max(cal * threads, 1e-9))))
+        t0 = time.perf_counter()
+        O.time_rfft_batch(xs[:one], 1)
+        dt1 = time.perf_counter() - t0
+        extra["cpu_1core_gbs"] = round(one * (4 * n + 8 * nb) / dt1 / 1e9, 3)
         del xs
     except Exception as e:                                     # report, do not hide
         cpu = {"value": None, "unit": "GB/s", "cores": None, "kind": "port",

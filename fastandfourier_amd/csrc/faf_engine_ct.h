@@ -8,7 +8,9 @@
  * carries __launch_bounds__ of its real block size, so the compiler keeps
  * every pass in registers (about 62 VGPRs instead of 128 plus spills).
  * Radix sequence for n = 2^a: 16, 16, ... then the leftover 2/4/8 last;
- * threads per signal T = n/16 (one radix-16 butterfly per thread).
+ * threads per signal T = n/16 (one radix-16 butterfly per thread).  The
+ * twiddle loads of all passes are issued at the start, next to the
+ * signal's HBM reads, so no pass waits on a table round trip.
  */
 #ifndef FAF_ENGINE_CT_H
 #define FAF_ENGINE_CT_H
@@ -16,52 +18,103 @@
 #define FAF_ENGINE_NO_PLAIN_KERNELS
 #include "faf_engine.h"
 
-template <int R, bool INV, int NS, int N, int T, class S, class D>
-FAF_DEV void stage_ct(const S &src, const D &dst, int lt, int active,
-                      const float2 *__restrict__ tw, bool sync) {
-    constexpr int NB = N / R;
-    constexpr int B = (NB + T - 1) / T;
-    float2 v[B][R];
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-        const int j = lt + b * T;
-        if (active && j < NB) {
-#pragma unroll
-            for (int t = 0; t < R; ++t) v[b][t] = src.ld(j + t * NB);
-        }
+/* Twiddles of a radix-R butterfly with step s are W^(t s), t = 1 .. R-1.
+ * Each butterfly loads only W^(s 2^i) (i < log2 R) from the FP64-built
+ * table and forms the others as products of at most four of them, so the
+ * loads of every later pass can be issued up front, beside the signal's own
+ * HBM reads, without holding R-1 values per pass in registers. */
+template <int R> struct TwBits { static constexpr int K = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : 1; };
+
+template <int R> FAF_DEV void tw_expand(const float2 *b, float2 *w) {
+    w[1] = b[0];
+    if constexpr (R > 2) {
+        w[2] = b[1];
+        w[3] = cmul(b[1], b[0]);
     }
-    if (sync) __syncthreads();
+    if constexpr (R > 4) {
+        w[4] = b[2];
+        w[5] = cmul(b[2], b[0]);
+        w[6] = cmul(b[2], b[1]);
+        w[7] = cmul(b[2], w[3]);
+    }
+    if constexpr (R > 8) {
+        w[8] = b[3];
 #pragma unroll
-    for (int b = 0; b < B; ++b) {
-        const int j = lt + b * T;
-        if (active && j < NB) {
-            const int k = j % NS;
-            if (NS > 1) {
-                const int step = k * (N / (NS * R));
-#pragma unroll
-                for (int t = 1; t < R; ++t) v[b][t] = cmul(v[b][t], tw[t * step]);
-            }
-            dft<R, INV>(v[b]);
-            const int base = (j - k) * R + k;
-#pragma unroll
-            for (int t = 0; t < R; ++t) dst.st(base + t * NS, v[b][t]);
-        }
+        for (int t = 9; t < 16; ++t) w[t] = cmul(b[3], w[t - 8]);
     }
 }
 
-/* all passes: src -> LDS ... LDS -> dst (callers barrier after an LDS sink) */
-template <bool INV, int N, int T, int NS, int R, int... Rest, class S, class D>
-FAF_DEV void ct_stages(const S &src, const Lds &lds, const D &dst, int lt, int active,
-                       const float2 *tw) {
-    if constexpr (sizeof...(Rest) == 0) {
-        stage_ct<R, INV, NS, N, T>(src, dst, lt, active, tw,
-                                   S::is_lds || (!S::is_lds && !D::is_lds));
-    } else {
-        stage_ct<R, INV, NS, N, T>(src, lds, lt, active, tw, S::is_lds);
-        __syncthreads();
-        ct_stages<INV, N, T, NS * R, Rest...>(lds, lds, dst, lt, active, tw);
+/* The passes of one length as a chain: pass (NS, R) then the rest.
+ * load_tw() issues the twiddle loads of every pass with span > 1; run()
+ * performs the passes src -> LDS ... LDS -> dst (callers barrier after an
+ * LDS sink). */
+template <bool INV, int N, int T, int NS, int... Rs> struct CtChain {
+    FAF_DEV void load_tw(int, int, const float2 *) {}
+};
+
+template <bool INV, int N, int T, int NS, int R, int... Rest>
+struct CtChain<INV, N, T, NS, R, Rest...> {
+    static constexpr int NB = N / R, B = (NB + T - 1) / T, K = TwBits<R>::K;
+    float2 tb[NS > 1 ? B : 1][NS > 1 ? K : 1];
+    CtChain<INV, N, T, NS * R, Rest...> next;
+
+    FAF_DEV void load_tw(int lt, int active, const float2 *__restrict__ tw) {
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int j = lt + b * T;
+                if (active && j < NB) {
+                    const int step = (j % NS) * (N / (NS * R));
+#pragma unroll
+                    for (int i = 0; i < K; ++i) tb[b][i] = tw[step << i];
+                }
+            }
+        }
+        next.load_tw(lt, active, tw);
     }
-}
+
+    template <class S, class D>
+    FAF_DEV void stage(const S &src, const D &dst, int lt, int active, bool sync) {
+        float2 v[B][R];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = lt + b * T;
+            if (active && j < NB) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) v[b][t] = src.ld(j + t * NB);
+            }
+        }
+        if (sync) __syncthreads();
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = lt + b * T;
+            if (active && j < NB) {
+                const int k = j % NS;
+                if constexpr (NS > 1) {
+                    float2 w[R];
+                    tw_expand<R>(tb[b], w);
+#pragma unroll
+                    for (int t = 1; t < R; ++t) v[b][t] = cmul(v[b][t], w[t]);
+                }
+                dft<R, INV>(v[b]);
+                const int base = (j - k) * R + k;
+#pragma unroll
+                for (int t = 0; t < R; ++t) dst.st(base + t * NS, v[b][t]);
+            }
+        }
+    }
+
+    template <class S, class D>
+    FAF_DEV void run(const S &src, const Lds &lds, const D &dst, int lt, int active) {
+        if constexpr (sizeof...(Rest) == 0) {
+            stage(src, dst, lt, active, S::is_lds || (!S::is_lds && !D::is_lds));
+        } else {
+            stage(src, lds, lt, active, S::is_lds);
+            __syncthreads();
+            next.run(lds, lds, dst, lt, active);
+        }
+    }
+};
 
 /* radix sequence of a power-of-two length, as a type */
 template <int... Rs> struct Radices {};
@@ -78,13 +131,10 @@ template <> struct Pow2Seq<16384> { using type = Radices<16, 16, 16, 4>; };
 template <bool INV, int N, class Seq> struct CtFFT;
 template <bool INV, int N, int... Rs> struct CtFFT<INV, N, Radices<Rs...>> {
     static constexpr int T = N / 16;
-    template <class S, class D>
-    static FAF_DEV void run(const S &src, const Lds &lds, const D &dst, int lt, int active,
-                            const float2 *tw) {
-        ct_stages<INV, N, T, 1, Rs...>(src, lds, dst, lt, active, tw);
-    }
+    using Chain = CtChain<INV, N, T, 1, Rs...>;
 };
 
+/* Ct<INV, N>::T threads per signal; Ct<INV, N>::Chain holds the passes */
 template <bool INV, int N> using Ct = CtFFT<INV, N, typename Pow2Seq<N>::type>;
 
 template <int T> constexpr int ct_spb() { return T >= 256 ? 1 : 256 / T; }
@@ -104,14 +154,16 @@ void kc_c2c(fafd_fft P, const float *in_re, const float *in_im, long in_dist, fl
     const long sig = active ? g : 0;
     Lds lds{faf_smem + sid * P.plane};
     const float2 *tw = (const float2 *)P.tw;
+    typename Ct<INV, N>::Chain ch;
+    ch.load_tw(lt, active, tw);
     if (INTER) {
         GInterIn src{(const float2 *)in_re + sig * in_dist};
         GInterOut dst{(float2 *)out_re + sig * out_dist, scale};
-        Ct<INV, N>::run(src, lds, dst, lt, active, tw);
+        ch.run(src, lds, dst, lt, active);
     } else {
         GSplitIn src{in_re + sig * in_dist, in_im + sig * in_dist};
         GSplitOut dst{out_re + sig * out_dist, out_im + sig * out_dist, scale};
-        Ct<INV, N>::run(src, lds, dst, lt, active, tw);
+        ch.run(src, lds, dst, lt, active);
     }
 }
 
@@ -126,7 +178,9 @@ void kc_r2c(fafd_fft P, const float2 *post, const float *x, long in_dist, float 
     const long sig = active ? g : 0;
     Lds lds{faf_smem + sid * P.plane};
     GPairIn src{x + sig * in_dist};
-    Ct<false, NC>::run(src, lds, lds, lt, active, (const float2 *)P.tw);
+    typename Ct<false, NC>::Chain ch;
+    ch.load_tw(lt, active, (const float2 *)P.tw);
+    ch.run(src, lds, lds, lt, active);
     __syncthreads();
     if (!active) return;
     float *ore = out_re + sig * out_dist;
@@ -165,6 +219,8 @@ void kc_c2r(fafd_fft P, const float2 *pre, const float *in_re, const float *in_i
     const int active = g < count;
     const long sig = active ? g : 0;
     Lds lds{faf_smem + sid * P.plane};
+    typename Ct<true, NC>::Chain ch;
+    ch.load_tw(lt, active, (const float2 *)P.tw);
     if (active) {
         const float *ire = in_re + sig * in_dist;
         const float *iim = in_im + sig * in_dist;
@@ -188,7 +244,7 @@ void kc_c2r(fafd_fft P, const float2 *pre, const float *in_re, const float *in_i
     }
     __syncthreads();
     GPairOut dst{y + sig * out_dist, scale};
-    Ct<true, NC>::run(lds, lds, dst, lt, active, (const float2 *)P.tw);
+    ch.run(lds, lds, dst, lt, active);
 }
 
 template <int NC>
@@ -202,7 +258,11 @@ void kc_pipe(fafd_fft Pf, fafd_fft Pi, const float2 *post, const float2 *pre, fa
     const long sig = active ? g : 0;
     Lds lds{faf_smem + sid * Pf.plane};
     GPairIn src{x + sig * in_dist};
-    Ct<false, NC>::run(src, lds, lds, lt, active, (const float2 *)Pf.tw);
+    typename Ct<false, NC>::Chain fw;
+    fw.load_tw(lt, active, (const float2 *)Pf.tw);
+    fw.run(src, lds, lds, lt, active);
+    typename Ct<true, NC>::Chain bw;
+    bw.load_tw(lt, active, (const float2 *)Pi.tw);
     __syncthreads();
     if (active) {
 #pragma unroll
@@ -229,7 +289,7 @@ void kc_pipe(fafd_fft Pf, fafd_fft Pi, const float2 *post, const float2 *pre, fa
     }
     __syncthreads();
     GPairOut dst{y + sig * out_dist, scale};
-    Ct<true, NC>::run(lds, lds, dst, lt, active, (const float2 *)Pi.tw);
+    bw.run(lds, lds, dst, lt, active);
 }
 
 template <int NC>
@@ -245,7 +305,9 @@ void kc_stft(fafd_fft P, const float2 *post, const float *win, const float *x, l
     Lds lds{faf_smem + sid * P.plane};
     const long rec = sig / frames, f = sig - rec * frames;
     GFrameIn src{x + rec * rec_dist + f * hop, win, rec_len - f * hop};
-    Ct<false, NC>::run(src, lds, lds, lt, active, (const float2 *)P.tw);
+    typename Ct<false, NC>::Chain ch;
+    ch.load_tw(lt, active, (const float2 *)P.tw);
+    ch.run(src, lds, lds, lt, active);
     __syncthreads();
     if (!active) return;
     float *ore = out_re + rec * rec_out_dist + f * frame_dist;

@@ -213,7 +213,7 @@ def test_serve_admission_waits_for_budget():
     say("case: serve admission budget")
     lib = G.L()
     rng = np.random.default_rng(14)
-    n = 16384
+    n = 4096            # short C2C: the result is all it holds (no workspace)
     one = lib.faf_server_result_bytes(None, 0, n)
     assert one == 0                                    # no server -> not servable
     s = server(hbm_budget=3 * (n * 8) + 4096, streams=2)
