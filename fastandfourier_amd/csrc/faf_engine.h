@@ -233,19 +233,43 @@ struct GPairOut {            /* C2R unpacking y[2i] = Re z, y[2i+1] = Im z */
 /* one radix-R Stockham pass                                           */
 /* ------------------------------------------------------------------ */
 
+/* Twiddles of a radix-R butterfly with step s are W^(t s), t = 1 .. R-1.
+ * A butterfly loads only W^(s 2^i) (i < K, 2^(K-1) <= R-1) from the
+ * FP64-built table and forms W^(t s) as the product of the loaded powers
+ * for the set bits of t (at most three products for t < 16).  The few
+ * loads go out with the pass's data reads, before its barrier. */
+template <int R> struct TwBits {
+    static constexpr int K = R <= 2 ? 1 : R <= 4 ? 2 : R <= 8 ? 3 : 4;
+};
+
+template <int R> FAF_DEV void tw_expand(const float2 *b, float2 *w) {
+#pragma unroll
+    for (int t = 1; t < R; ++t) {
+        const int i = __builtin_ctz(t);
+        w[t] = t == (1 << i) ? b[i] : cmul(w[t & (t - 1)], b[i]);
+    }
+}
+
 template <int R, bool INV, class S, class D>
 FAF_DEV void stage(const S &src, const D &dst, int n, int active, int lt, int T,
                    const fafd_stage &st, const float2 *__restrict__ tw, bool sync) {
-    constexpr int B = FAFD_MAXE / R;
+    constexpr int B = FAFD_MAXE / R, K = TwBits<R>::K;
     const int nb = n / R;
     const int nbe = active ? nb : 0;
     float2 v[B][R];
+    float2 tb[B][K];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = lt + b * T;
         if (j < nbe) {
 #pragma unroll
             for (int t = 0; t < R; ++t) v[b][t] = src.ld(j + t * nb);
+            if (st.ns > 1) {
+                const int k = j - (int)__umulhi((unsigned)j, st.ns_magic) * st.ns;
+                const int step = k * st.tw_stride;
+#pragma unroll
+                for (int i = 0; i < K; ++i) tb[b][i] = tw[step << i];
+            }
         }
     }
     if (sync) __syncthreads();
@@ -256,13 +280,10 @@ FAF_DEV void stage(const S &src, const D &dst, int n, int active, int lt, int T,
             int k = 0;
             if (st.ns > 1) {
                 k = j - (int)__umulhi((unsigned)j, st.ns_magic) * st.ns;
-                const int step = k * st.tw_stride;
-                int idx = step;
+                float2 w[R];
+                tw_expand<R>(tb[b], w);
 #pragma unroll
-                for (int t = 1; t < R; ++t) {
-                    v[b][t] = cmul(v[b][t], tw[idx]);
-                    idx += step;
-                }
+                for (int t = 1; t < R; ++t) v[b][t] = cmul(v[b][t], w[t]);
             }
             dft<R, INV>(v[b]);
             const int base = (j - k) * R + k;

@@ -18,32 +18,9 @@
 #define FAF_ENGINE_NO_PLAIN_KERNELS
 #include "faf_engine.h"
 
-/* Twiddles of a radix-R butterfly with step s are W^(t s), t = 1 .. R-1.
- * Each butterfly loads only W^(s 2^i) (i < log2 R) from the FP64-built
- * table and forms the others as products of at most four of them, so the
- * loads of every later pass can be issued up front, beside the signal's own
- * HBM reads, without holding R-1 values per pass in registers. */
-template <int R> struct TwBits { static constexpr int K = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : 1; };
-
-template <int R> FAF_DEV void tw_expand(const float2 *b, float2 *w) {
-    w[1] = b[0];
-    if constexpr (R > 2) {
-        w[2] = b[1];
-        w[3] = cmul(b[1], b[0]);
-    }
-    if constexpr (R > 4) {
-        w[4] = b[2];
-        w[5] = cmul(b[2], b[0]);
-        w[6] = cmul(b[2], b[1]);
-        w[7] = cmul(b[2], w[3]);
-    }
-    if constexpr (R > 8) {
-        w[8] = b[3];
-#pragma unroll
-        for (int t = 9; t < 16; ++t) w[t] = cmul(b[3], w[t - 8]);
-    }
-}
-
+/* The twiddles of each butterfly come from the loaded powers W^(s 2^i)
+ * (TwBits / tw_expand in faf_engine.h), so every pass's loads can be issued
+ * up front, beside the signal's own HBM reads. */
 /* The passes of one length as a chain: pass (NS, R) then the rest.
  * load_tw() issues the twiddle loads of every pass with span > 1; run()
  * performs the passes src -> LDS ... LDS -> dst (callers barrier after an
