@@ -444,7 +444,9 @@ int fafi_cwt_build(fafh_plan *p, const faf_cwt_config *cfg, int inverse_kind) {
         p->sub_c2r = p->sub_c2c ? cwt_sub(n, 1, 1, p->scale) : NULL;
         return p->sub_c2r ? 0 : -1;
     }
-    if (!fafi_radix_fits(nc, 0) || fafi_make_fft(nc, 0, 0, NULL, 0, &p->a)) {
+    /* the inverse holds a row spectrum and the accumulator side by side */
+    if (!fafi_radix_fits(nc, 0) || 2 * (nc + (nc >> 4) + 1) * 8 > FAFD_MAX_LDS_BYTES ||
+        fafi_make_fft(nc, 0, 0, NULL, 0, &p->a)) {
         fafi_err("cwt: inverse of length %zu does not fit one workgroup", n);
         return -1;
     }
