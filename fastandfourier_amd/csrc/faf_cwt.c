@@ -435,6 +435,7 @@ int fafi_cwt_build(fafh_plan *p, const faf_cwt_config *cfg, int inverse_kind) {
             if (fafi_make_fft(nc, 0, 0, NULL, 0, &p->a) ||
                 fafi_make_fft(n, 1, 0, NULL, 0, &p->full_inv))
                 return -1;
+            p->ct = fafi_ct_enabled() && fafk_ct_supports(5, (int)n);
             return 0;
         }
         /* longer segments: rfft(n), spread the filtered rows, then one
@@ -566,8 +567,15 @@ int fafi_run_cwt(fafh_plan *p, const float *x, long idist, float *out, long odis
     if (groups < 1) groups = 1;
     if (groups > rows) groups = rows;
     int rpw = (rows + groups - 1) / groups;
-    int rc = fafk_cwt(&p->a, &p->full_inv, !complex_out, p->post, p->psi_dev, p->psi_stride,
-                      rows, rpw, x, idist, out, odist, count, p->scale, stream);
+    int rc;
+    if (p->ct) {
+        rc = fafk_ct_cwt(&p->a, &p->full_inv, !complex_out, p->post, p->psi_dev, p->psi_stride,
+                         rows, rpw, x, idist, out, odist, count, p->scale, stream);
+        if (rc) fafi_err("%s", fafk_ct_last_error());
+        return rc;
+    }
+    rc = fafk_cwt(&p->a, &p->full_inv, !complex_out, p->post, p->psi_dev, p->psi_stride, rows,
+                  rpw, x, idist, out, odist, count, p->scale, stream);
     if (rc) fafi_err("%s", fafk_last_error());
     return rc;
 }

@@ -307,4 +307,58 @@ void kc_stft(fafd_fft P, const float2 *post, const float *win, const float *x, l
     }
 }
 
+/* CWT bank on one segment per workgroup (rows split over workgroups when
+ * segments are few): rfft(N) into X, then for each row r the one-sided
+ * X psi_r through an inverse N-point pass into W and out.  Both passes'
+ * twiddles are loaded once and serve every row of the workgroup. */
+template <bool REAL_OUT, int N>
+__global__ __launch_bounds__(N / 16)
+void kc_cwt(fafd_fft Ph, fafd_fft Pn, const float2 *post, const float *psi, long psi_stride,
+            int rows, int rows_per_wg, const float *x, long in_dist, float *out,
+            long out_dist, long count, float scale) {
+    constexpr int NC = N / 2, TH = NC / 16;
+    const int groups = (rows + rows_per_wg - 1) / rows_per_wg;
+    const long seg = blockIdx.x / groups;
+    const int g = (int)(blockIdx.x - seg * groups);
+    const int tid = (int)threadIdx.x;
+    const int ah = tid < TH && seg < count, an = seg < count;
+    const int lh = tid < TH ? tid : 0;
+    Lds X{faf_smem};
+    Lds W{faf_smem + Ph.plane};
+    typename Ct<false, NC>::Chain fh;
+    typename Ct<true, N>::Chain fn;
+    fh.load_tw(lh, ah, (const float2 *)Ph.tw);
+    fn.load_tw(tid, an, (const float2 *)Pn.tw);
+    fh.run(GPairIn{x + (seg < count ? seg : 0) * in_dist}, X, X, lh, ah);
+    __syncthreads();
+    if (ah) {
+#pragma unroll
+        for (int i = 0; i < (NC / 2 + TH) / TH; ++i) {
+            const int k = tid + i * TH;
+            if (k > NC / 2) break;
+            const int m = NC - k;
+            float2 xk, xm;
+            if (k == 0) {
+                float2 z = X.ld(0);
+                xk = make_float2(z.x + z.y, 0.0f);
+                xm = make_float2(z.x - z.y, 0.0f);
+            } else {
+                kiss_post(X.ld(k), X.ld(m), post[k], xk, xm);
+            }
+            X.st(k, xk);
+            X.st(m, xm);
+        }
+    }
+    __syncthreads();
+    const int r0 = g * rows_per_wg;
+    const int r1 = r0 + rows_per_wg < rows ? r0 + rows_per_wg : rows;
+    for (int r = r0; r < r1; ++r) {
+        OneSidedIn src{X, psi + (long)r * psi_stride, NC};
+        const long o = (seg < count ? seg : 0) * out_dist + (long)r * N;
+        if (REAL_OUT) fn.run(src, W, GRealOnlyOut{out + o, scale}, tid, an);
+        else fn.run(src, W, GInterOut{(float2 *)out + o, scale}, tid, an);
+        __syncthreads();
+    }
+}
+
 #endif

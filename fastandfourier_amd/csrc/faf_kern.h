@@ -105,6 +105,9 @@ int fafk_ct_c2r(const fafd_fft *P, int inter_in, const void *pre, const float *i
 int fafk_ct_pipe(const fafd_fft *Pf, const fafd_fft *Pi, const void *post, const void *pre,
                  const fafd_ops *ops, const float *x, long in_dist, float *y, long out_dist,
                  long count, float scale, void *stream);
+int fafk_ct_cwt(const fafd_fft *Ph, const fafd_fft *Pn, int real_out, const void *post,
+                const float *psi, long psi_stride, int rows, int rows_per_wg, const float *x,
+                long in_dist, float *out, long out_dist, long count, float scale, void *stream);
 int fafk_ct_stft(const fafd_fft *P, const void *post, const float *win, const float *x,
                  long rec_len, long rec_dist, long hop, long frames, float *out_re,
                  float *out_im, long frame_dist, long rec_out_dist, long count, void *stream);

@@ -25,7 +25,7 @@ static inline bool ct_len(int n) {
 }
 
 extern "C" int fafk_ct_supports(int kind, int n) {
-    (void)kind;
+    if (kind == 5) return n >= 512 && n <= 8192 && ct_len(n);   /* CWT segment length */
     return ct_len(n) ? 1 : 0;
 }
 
@@ -46,7 +46,13 @@ template <int N> static void ct_attrs_n() {
     ct_attr(kc_pipe<N>); ct_attr(kc_stft<N>);
 }
 
+template <int N> static void ct_attrs_cwt() {
+    ct_attr(kc_cwt<false, N>); ct_attr(kc_cwt<true, N>);
+}
+
 extern "C" int fafk_ct_init(void) {
+    ct_attrs_cwt<512>(); ct_attrs_cwt<1024>(); ct_attrs_cwt<2048>(); ct_attrs_cwt<4096>();
+    ct_attrs_cwt<8192>();
     ct_attrs_n<256>(); ct_attrs_n<512>(); ct_attrs_n<1024>(); ct_attrs_n<2048>();
     ct_attrs_n<4096>(); ct_attrs_n<8192>(); ct_attrs_n<16384>();
     return 0;
@@ -142,4 +148,33 @@ extern "C" int fafk_ct_stft(const fafd_fft *P, const void *post, const float *wi
 #undef STFT
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : ct_fail("kc_stft", e);
+}
+
+/* CWT bank rows of power-of-two segments 512 ... 8192 (kind 5) */
+extern "C" int fafk_ct_cwt(const fafd_fft *Ph, const fafd_fft *Pn, int real_out, const void *post,
+                           const float *psi, long psi_stride, int rows, int rows_per_wg,
+                           const float *x, long in_dist, float *out, long out_dist, long count,
+                           float scale, void *stream) {
+    const int groups = (rows + rows_per_wg - 1) / rows_per_wg;
+    dim3 g((unsigned)(count * groups));
+    const size_t lds = (size_t)(Ph->plane + Pn->plane) * sizeof(float2);
+    const float2 *po = (const float2 *)post;
+#define CWT(N)                                                                           \
+    do {                                                                                 \
+        if (real_out) hipLaunchKernelGGL((kc_cwt<true, N>), g, dim3(N / 16), lds, S(stream), *Ph, *Pn, po, psi, psi_stride, rows, rows_per_wg, x, in_dist, out, out_dist, count, scale); \
+        else hipLaunchKernelGGL((kc_cwt<false, N>), g, dim3(N / 16), lds, S(stream), *Ph, *Pn, po, psi, psi_stride, rows, rows_per_wg, x, in_dist, out, out_dist, count, scale); \
+    } while (0)
+    switch (Pn->n) {
+    case 512: CWT(512); break;
+    case 1024: CWT(1024); break;
+    case 2048: CWT(2048); break;
+    case 4096: CWT(4096); break;
+    case 8192: CWT(8192); break;
+    default:
+        snprintf(g_ct_err, sizeof g_ct_err, "no specialised CWT kernel for length %d", Pn->n);
+        return -1;
+    }
+#undef CWT
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : ct_fail("kc_cwt", e);
 }

@@ -10,7 +10,13 @@
 #include "faf_internal.h"
 #include <string.h>
 
-#define CHUNK_BYTES ((size_t)256 << 20)   /* workspace per four-step chunk */
+/* workspace per four-step chunk: small enough that the intermediate the
+ * column pass writes is still in the 256 MB Infinity Cache when the row
+ * pass reads it (measured: 64 MB beats 256 MB by 27% and 16 MB, whose
+ * launches are too short, by 65%) */
+#ifndef CHUNK_BYTES
+#define CHUNK_BYTES ((size_t)64 << 20)
+#endif
 
 static int kerr(int rc) {
     if (rc) fafi_err("%s", fafk_last_error());
