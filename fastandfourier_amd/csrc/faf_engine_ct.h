@@ -97,6 +97,7 @@ struct CtChain<INV, N, T, NS, R, Rest...> {
 template <int... Rs> struct Radices {};
 
 template <int N> struct Pow2Seq;
+template <> struct Pow2Seq<128> { using type = Radices<16, 8>; };
 template <> struct Pow2Seq<256> { using type = Radices<16, 16>; };
 template <> struct Pow2Seq<512> { using type = Radices<16, 16, 2>; };
 template <> struct Pow2Seq<1024> { using type = Radices<16, 16, 4>; };
@@ -358,6 +359,187 @@ void kc_cwt(fafd_fft Ph, fafd_fft Pn, const float2 *post, const float *psi, long
         if (REAL_OUT) fn.run(src, W, GRealOnlyOut{out + o, scale}, tid, an);
         else fn.run(src, W, GInterOut{(float2 *)out + o, scale}, tid, an);
         __syncthreads();
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* four-step for power-of-two lengths beyond one workgroup, n = N1 N2   */
+/* ------------------------------------------------------------------ */
+
+/* what the column pass reads: complex pairs of a real row (R2C), complex
+ * (split or interleaved), or the Kiss pre-pass of a Hermitian spectrum
+ * (C2R, split or interleaved) */
+enum { F4_IN_PAIR = 0, F4_IN_SPLIT = 1, F4_IN_INTER = 2, F4_PRE_SPLIT = 3, F4_PRE_INTER = 4 };
+/* what the row pass writes: complex (split / interleaved, natural order),
+ * the real pairs of a C2R row, or the Kiss post-pass bins of an R2C row */
+enum { F4_OUT_SPLIT = 0, F4_OUT_INTER = 1, F4_OUT_PAIR = 2, F4_POST_SPLIT = 3, F4_POST_INTER = 4 };
+
+constexpr int F4_COLS = 32;   /* columns per column-pass workgroup */
+constexpr int F4_ROWS = 32;   /* row-pass blocks hold 2 F4_ROWS rows (measured: 32 beats 16) */
+
+/* element i of a Hermitian spectrum with nc+1 bins (split or interleaved) */
+template <bool INTER> FAF_DEV float2 herm_ld(const float *re, const float *im, long i) {
+    return INTER ? ((const float2 *)re)[i] : make_float2(re[i], im[i]);
+}
+
+/* Column pass: columns c0 .. c0+COLS of the [N1][N2] view, FFT N1, times
+ * W_n^(k1 c), into ws[k1][N2] (interleaved).  in_dist in floats, ws_dist
+ * in float2.  grid (N2/COLS, signals). */
+template <int IN, bool INV, int N1, int N2>
+__global__ __launch_bounds__(F4_COLS * (N1 / 16))
+void kc4_col(fafd_fft P1, const float2 *twn, const void *aux, const float *in_re,
+             const float *in_im, long in_dist, float2 *ws, long ws_dist) {
+    constexpr int T = N1 / 16, NC = N1 * N2, PL = N1 + N1 / 16 + 1;
+    const long b = blockIdx.y;
+    const int c0 = blockIdx.x * F4_COLS, tid = (int)threadIdx.x;
+    const int sid = tid / T, lt = tid - sid * T;
+    typename Ct<INV, N1>::Chain ch;
+    ch.load_tw(lt, 1, (const float2 *)P1.tw);
+    const float *xr = in_re + b * in_dist;
+    const float *xi = in_im ? in_im + b * in_dist : nullptr;
+    for (int f = tid; f < F4_COLS * N1; f += F4_COLS * T) {
+        const int c = f % F4_COLS, r = f / F4_COLS;
+        const long i = (long)r * N2 + c0 + c;
+        float2 v;
+        if (IN == F4_IN_PAIR) {
+            v = ((const float2 *)xr)[i];
+        } else if (IN == F4_IN_SPLIT) {
+            v = make_float2(xr[i], xi[i]);
+        } else if (IN == F4_IN_INTER) {
+            v = ((const float2 *)xr)[i];
+        } else {                      /* Kiss pre-pass, F[i] from X[i], X[NC - i] */
+            constexpr bool HI = IN == F4_PRE_INTER;
+            const float2 *pre = (const float2 *)aux;
+            if (i == 0) {
+                const float2 a = herm_ld<HI>(xr, xi, 0), e = herm_ld<HI>(xr, xi, NC);
+                v = make_float2(a.x + e.x, a.x - e.x);
+            } else if (i <= NC / 2) {
+                float2 fk, fm;
+                kiss_pre(herm_ld<HI>(xr, xi, i), herm_ld<HI>(xr, xi, NC - i), pre[i], fk, fm);
+                v = fk;
+            } else {
+                float2 fk, fm;
+                kiss_pre(herm_ld<HI>(xr, xi, NC - i), herm_ld<HI>(xr, xi, i), pre[NC - i], fk, fm);
+                v = fm;
+            }
+        }
+        Lds{faf_smem + c * PL}.st(r, v);
+    }
+    __syncthreads();
+    Lds lds{faf_smem + sid * PL};
+    ch.run(lds, lds, lds, lt, 1);
+    __syncthreads();
+    float2 *w = ws + b * ws_dist;
+    for (int f = tid; f < F4_COLS * N1; f += F4_COLS * T) {
+        const int c = f % F4_COLS, k1 = f / F4_COLS;
+        const float2 v = Lds{faf_smem + c * PL}.ld(k1);
+        w[(long)k1 * N2 + c0 + c] = cmul(v, twn[(long)k1 * (c0 + c)]);
+    }
+}
+
+/* Row pass (all row distances in floats): rows of ws [N1][N2], FFT N2,
+ * results Z[k1 + N1 k2].  For the
+ * R2C post-pass a workgroup holds a block A of rows k1 in [0, N1/2] and
+ * their mirrors N1 - k1, so the partner Z[nc - k] of every bin it writes
+ * is in its LDS: X[k] = post(Z[k], Z[nc - k]) is written straight to the
+ * Hermitian output and the packed spectrum never goes back to HBM.
+ * Other outputs use plain blocks of ROWS rows.  grid (blocks, signals). */
+template <int OUT, bool INV, int N1, int N2, int RW>
+__global__ __launch_bounds__(2 * RW * (N2 / 16))
+void kc4_row(fafd_fft P2, const float2 *ws, long ws_dist, const float2 *post, float *out_re,
+             float *out_im, long out_dist, float scale) {
+    constexpr int T = N2 / 16, NC = N1 * N2, PL = N2 + N2 / 16 + 1;
+    constexpr bool POST = OUT == F4_POST_SPLIT || OUT == F4_POST_INTER;
+    __shared__ int s_row[2 * RW];
+    __shared__ int s_slot_of_mirror[2 * RW];
+    __shared__ int s_nrows;
+    const long b = blockIdx.y;
+    const int tid = (int)threadIdx.x, bd = (int)blockDim.x;
+    if (tid == 0) {
+        int n = 0;
+        if (POST) {
+            /* block x holds A = [x RW, x RW + RW) (block 0 also N1/2) and
+             * the mirrors N1 - a of a != 0, N1/2: 2 RW rows every block */
+            const int a0 = blockIdx.x * RW;
+            for (int a = a0; a < a0 + RW; ++a) s_row[n++] = a;
+            if (blockIdx.x == 0) s_row[n++] = N1 / 2;
+            const int na = n;
+            for (int i = 0; i < na; ++i) {
+                const int a = s_row[i];
+                if (a == 0 || 2 * a == N1) {
+                    s_slot_of_mirror[i] = i;
+                } else {
+                    s_row[n] = N1 - a;
+                    s_slot_of_mirror[i] = n;
+                    s_slot_of_mirror[n] = i;
+                    ++n;
+                }
+            }
+        } else {
+            const int a0 = blockIdx.x * 2 * RW;
+            for (int a = a0; a < a0 + 2 * RW && a < N1; ++a) s_row[n++] = a;
+        }
+        s_nrows = n;
+    }
+    __syncthreads();
+    const int nrows = s_nrows;
+    const int sid = tid / T, lt = tid - sid * T;
+    typename Ct<INV, N2>::Chain ch;
+    ch.load_tw(lt, sid < nrows, (const float2 *)P2.tw);
+    const float2 *w = ws + b * ws_dist;
+    for (int f = tid; f < nrows * N2; f += bd) {
+        const int s = f / N2, c = f - s * N2;
+        Lds{faf_smem + s * PL}.st(c, w[(long)s_row[s] * N2 + c]);
+    }
+    __syncthreads();
+    Lds lds{faf_smem + sid * PL};
+    ch.run(lds, lds, lds, lt, sid < nrows);
+    __syncthreads();
+    /* k2-major order: consecutive threads take consecutive rows, so each
+     * store instruction covers runs of adjacent output bins */
+    for (int f = tid; f < nrows * N2; f += bd) {
+        const int s = f % nrows, k2 = f / nrows;
+        const int r = s_row[s];
+        const long k = (long)r + (long)N1 * k2;
+        const float2 z = Lds{faf_smem + s * PL}.ld(k2);
+        if (POST) {
+            /* partner Z[nc - k]: row (N1 - r) mod N1, column N2-1-k2, or
+             * N2-k2 (mod N2) on row 0 */
+            const int ms = s_slot_of_mirror[s];
+            const int mk2 = r == 0 ? (N2 - k2) & (N2 - 1) : N2 - 1 - k2;
+            const float2 zm = Lds{faf_smem + ms * PL}.ld(mk2);
+            float2 xk, xm;
+            if (k == 0) {
+                xk = make_float2(z.x + z.y, 0.0f);
+                xm = make_float2(z.x - z.y, 0.0f);   /* Nyquist bin NC */
+            } else if (k <= NC / 2) {
+                kiss_post(z, zm, post[k], xk, xm);
+            } else {
+                float2 t;
+                kiss_post(zm, z, post[NC - k], t, xk);
+            }
+            xk = cscale(xk, scale);
+            if (OUT == F4_POST_INTER) {
+                float2 *o = (float2 *)(out_re + b * out_dist);
+                o[k] = xk;
+                if (k == 0) o[NC] = cscale(xm, scale);
+            } else {
+                out_re[b * out_dist + k] = xk.x;
+                out_im[b * out_dist + k] = xk.y;
+                if (k == 0) {
+                    out_re[b * out_dist + NC] = xm.x * scale;
+                    out_im[b * out_dist + NC] = 0.0f;
+                }
+            }
+        } else {
+            const float2 v = cscale(z, scale);
+            if (OUT == F4_OUT_INTER || OUT == F4_OUT_PAIR)
+                ((float2 *)(out_re + b * out_dist))[k] = v;
+            else {
+                out_re[b * out_dist + k] = v.x;
+                out_im[b * out_dist + k] = v.y;
+            }
+        }
     }
 }
 

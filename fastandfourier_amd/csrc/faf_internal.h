@@ -62,6 +62,7 @@ typedef struct fafh_plan {
     void *blue_mem[2];
     int spb;
     int ct;                   /* use the compile-time specialised kernel */
+    int ct4;                  /* four-step with specialised column/row passes */
     float scale;
     int interleaved;          /* buffers are interleaved complex */
     void *ws;                 /* device workspace */

@@ -112,6 +112,18 @@ int fafk_ct_stft(const fafd_fft *P, const void *post, const float *win, const fl
                  long rec_len, long rec_dist, long hop, long frames, float *out_re,
                  float *out_im, long frame_dist, long rec_out_dist, long count, void *stream);
 
+/* specialised four-step (n1 x n2 = 128 x 256 or 256 x 256): kinds of the
+ * engine's F4_IN_* / F4_OUT_* enums */
+enum { FAFK_F4_IN_PAIR = 0, FAFK_F4_IN_SPLIT = 1, FAFK_F4_IN_INTER = 2, FAFK_F4_PRE_SPLIT = 3,
+       FAFK_F4_PRE_INTER = 4 };
+enum { FAFK_F4_OUT_SPLIT = 0, FAFK_F4_OUT_INTER = 1, FAFK_F4_OUT_PAIR = 2,
+       FAFK_F4_POST_SPLIT = 3, FAFK_F4_POST_INTER = 4 };
+int fafk_ct4_supported(int n1, int n2);
+int fafk_ct4(int in_kind, int out_kind, int inv, const fafd_fft *P1, const fafd_fft *P2,
+             const void *twn, const void *pre, const void *post, const float *in_re,
+             const float *in_im, long in_dist, void *ws, long ws_dist, float *out_re,
+             float *out_im, long out_dist, long count, float scale, void *stream);
+
 /* four-step I/O kinds (match the engine's enums) */
 enum { FAFK_FS_IN_SPLIT = 0, FAFK_FS_IN_PAIR = 1, FAFK_FS_IN_INTER = 2 };
 enum { FAFK_FS_OUT_SPLIT = 0, FAFK_FS_OUT_PAIR = 1, FAFK_FS_OUT_INTER = 2 };

@@ -50,7 +50,26 @@ template <int N> static void ct_attrs_cwt() {
     ct_attr(kc_cwt<false, N>); ct_attr(kc_cwt<true, N>);
 }
 
+template <int N1, int N2, int RW> static void ct_attrs_4r() {
+    ct_attr(kc4_row<F4_POST_SPLIT, false, N1, N2, RW>);
+    ct_attr(kc4_row<F4_POST_INTER, false, N1, N2, RW>);
+    ct_attr(kc4_row<F4_OUT_SPLIT, false, N1, N2, RW>);
+    ct_attr(kc4_row<F4_OUT_SPLIT, true, N1, N2, RW>);
+    ct_attr(kc4_row<F4_OUT_INTER, false, N1, N2, RW>);
+    ct_attr(kc4_row<F4_OUT_INTER, true, N1, N2, RW>);
+    ct_attr(kc4_row<F4_OUT_PAIR, true, N1, N2, RW>);
+}
+
+template <int N1, int N2> static void ct_attrs_4() {
+    ct_attr(kc4_col<F4_IN_PAIR, false, N1, N2>);
+    ct_attr(kc4_col<F4_IN_SPLIT, false, N1, N2>); ct_attr(kc4_col<F4_IN_SPLIT, true, N1, N2>);
+    ct_attr(kc4_col<F4_IN_INTER, false, N1, N2>); ct_attr(kc4_col<F4_IN_INTER, true, N1, N2>);
+    ct_attr(kc4_col<F4_PRE_SPLIT, true, N1, N2>); ct_attr(kc4_col<F4_PRE_INTER, true, N1, N2>);
+    ct_attrs_4r<N1, N2, F4_ROWS>();
+}
+
 extern "C" int fafk_ct_init(void) {
+    ct_attrs_4<128, 256>(); ct_attrs_4<256, 256>();
     ct_attrs_cwt<512>(); ct_attrs_cwt<1024>(); ct_attrs_cwt<2048>(); ct_attrs_cwt<4096>();
     ct_attrs_cwt<8192>();
     ct_attrs_n<256>(); ct_attrs_n<512>(); ct_attrs_n<1024>(); ct_attrs_n<2048>();
@@ -177,4 +196,84 @@ extern "C" int fafk_ct_cwt(const fafd_fft *Ph, const fafd_fft *Pn, int real_out,
 #undef CWT
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : ct_fail("kc_cwt", e);
+}
+
+/* ---------------- four-step, power-of-two n1 x n2 ---------------- */
+
+template <int N1, int N2, int RW>
+static int ct4_row(int out_kind, int inv, const fafd_fft *P2, float2 *w, long ws_dist,
+                   const void *post, float *out_re, float *out_im, long out_dist, long count,
+                   float scale, hipStream_t st);
+
+extern "C" int fafk_ct4_supported(int n1, int n2) {
+    return (n1 == 128 && n2 == 256) || (n1 == 256 && n2 == 256);
+}
+
+template <int N1, int N2>
+static int ct4_go(int in_kind, int out_kind, int inv, const fafd_fft *P1, const fafd_fft *P2,
+                  const void *twn, const void *pre, const void *post, const float *in_re,
+                  const float *in_im, long in_dist, void *ws, long ws_dist, float *out_re,
+                  float *out_im, long out_dist, long count, float scale, hipStream_t st) {
+    const dim3 gc(N2 / F4_COLS, (unsigned)count), bc(F4_COLS * (N1 / 16));
+    const size_t lc = (size_t)F4_COLS * (N1 + N1 / 16 + 1) * sizeof(float2);
+    const float2 *tw = (const float2 *)twn;
+    float2 *w = (float2 *)ws;
+#define COL(I, V) hipLaunchKernelGGL((kc4_col<I, V, N1, N2>), gc, bc, lc, st, *P1, tw, pre, in_re, in_im, in_dist, w, ws_dist)
+    switch (in_kind) {
+    case F4_IN_PAIR: COL(F4_IN_PAIR, false); break;
+    case F4_IN_SPLIT: if (inv) COL(F4_IN_SPLIT, true); else COL(F4_IN_SPLIT, false); break;
+    case F4_IN_INTER: if (inv) COL(F4_IN_INTER, true); else COL(F4_IN_INTER, false); break;
+    case F4_PRE_SPLIT: COL(F4_PRE_SPLIT, true); break;
+    case F4_PRE_INTER: COL(F4_PRE_INTER, true); break;
+    default: snprintf(g_ct_err, sizeof g_ct_err, "four-step input kind %d", in_kind); return -1;
+    }
+#undef COL
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return ct_fail("kc4_col", e);
+    return ct4_row<N1, N2, F4_ROWS>(out_kind, inv, P2, w, ws_dist, post, out_re, out_im,
+                                    out_dist, count, scale, st);
+}
+
+template <int N1, int N2, int RW>
+static int ct4_row(int out_kind, int inv, const fafd_fft *P2, float2 *w, long ws_dist,
+                   const void *post, float *out_re, float *out_im, long out_dist, long count,
+                   float scale, hipStream_t st) {
+    /* post-pass blocks hold 2 RW rows (RW rows of [0, N1/2) and their
+     * mirrors, N1/2 in block 0); plain blocks 2 RW consecutive rows */
+    const int blocks = N1 / (2 * RW);
+    const dim3 gr((unsigned)blocks, (unsigned)count), br(2 * RW * (N2 / 16));
+    const size_t lr = (size_t)2 * RW * (N2 + N2 / 16 + 1) * sizeof(float2);
+    const float2 *po = (const float2 *)post;
+#define ROW(O, V) hipLaunchKernelGGL((kc4_row<O, V, N1, N2, RW>), gr, br, lr, st, *P2, (const float2 *)w, ws_dist, po, out_re, out_im, out_dist, scale)
+    switch (out_kind) {
+    case F4_POST_SPLIT: ROW(F4_POST_SPLIT, false); break;
+    case F4_POST_INTER: ROW(F4_POST_INTER, false); break;
+    case F4_OUT_SPLIT: if (inv) ROW(F4_OUT_SPLIT, true); else ROW(F4_OUT_SPLIT, false); break;
+    case F4_OUT_INTER: if (inv) ROW(F4_OUT_INTER, true); else ROW(F4_OUT_INTER, false); break;
+    case F4_OUT_PAIR: ROW(F4_OUT_PAIR, true); break;
+    default: snprintf(g_ct_err, sizeof g_ct_err, "four-step output kind %d", out_kind); return -1;
+    }
+#undef ROW
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : ct_fail("kc4_row", e);
+}
+
+/* column pass then row pass of `count` signals; dists in floats for the
+ * input and output planes (interleaved rows count two floats a bin) and in
+ * float2 for the workspace */
+extern "C" int fafk_ct4(int in_kind, int out_kind, int inv, const fafd_fft *P1,
+                        const fafd_fft *P2, const void *twn, const void *pre, const void *post,
+                        const float *in_re, const float *in_im, long in_dist, void *ws,
+                        long ws_dist, float *out_re, float *out_im, long out_dist, long count,
+                        float scale, void *stream) {
+    if (P1->n == 128 && P2->n == 256)
+        return ct4_go<128, 256>(in_kind, out_kind, inv, P1, P2, twn, pre, post, in_re, in_im,
+                                in_dist, ws, ws_dist, out_re, out_im, out_dist, count, scale,
+                                S(stream));
+    if (P1->n == 256 && P2->n == 256)
+        return ct4_go<256, 256>(in_kind, out_kind, inv, P1, P2, twn, pre, post, in_re, in_im,
+                                in_dist, ws, ws_dist, out_re, out_im, out_dist, count, scale,
+                                S(stream));
+    snprintf(g_ct_err, sizeof g_ct_err, "no specialised four-step for %d x %d", P1->n, P2->n);
+    return -1;
 }

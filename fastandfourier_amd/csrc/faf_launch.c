@@ -41,6 +41,8 @@ size_t fafi_run_ws_bytes(const fafh_plan *p, long count) {
         return (size_t)chunk_signals(p->n, count) * p->n * 2 * sizeof(float);
     case PK_R2C_4S:
     case PK_C2R_4S:
+        if (p->ct4)
+            return (size_t)chunk_signals(p->n / 2, count) * (p->n / 2) * 2 * sizeof(float);
         return (size_t)chunk_signals(p->n / 2, count) * (p->n / 2) * 4 * sizeof(float);
     case PK_CWT:
         return fafi_cwt_ws_bytes(p, count, 1);
@@ -80,6 +82,19 @@ int fafi_run_c2c(fafh_plan *p, const float *ir, const float *ii, long idist, flo
         long cs = chunk_signals(n, count);
         float *ws = fafi_workspace(p, (size_t)cs * n * 2 * sizeof(float));
         if (!ws) return -1;
+        if (p->ct4) {
+            const long fi = idist * (inter ? 2 : 1), fo = odist * (inter ? 2 : 1);
+            for (long b0 = 0; b0 < count; b0 += cs) {
+                long c = count - b0 < cs ? count - b0 : cs;
+                int rc = fafk_ct4(inter ? FAFK_F4_IN_INTER : FAFK_F4_IN_SPLIT,
+                                  inter ? FAFK_F4_OUT_INTER : FAFK_F4_OUT_SPLIT, p->inverse,
+                                  &p->a, &p->b, p->twn, NULL, NULL, ir + b0 * fi,
+                                  inter ? NULL : ii + b0 * fi, fi, ws, (long)n, orr + b0 * fo,
+                                  inter ? NULL : oi + b0 * fo, fo, c, p->scale, stream);
+                if (kcerr(rc)) return -1;
+            }
+            return 0;
+        }
         float *tr = ws, *ti = ws + (size_t)cs * n;
         for (long b0 = 0; b0 < count; b0 += cs) {
             long c = count - b0 < cs ? count - b0 : cs;
@@ -114,6 +129,21 @@ int fafi_run_r2c(fafh_plan *p, const float *x, long idist, float *orr, float *oi
     if (p->kind != PK_R2C_4S) { fafi_err("plan is not an R2C transform"); return -1; }
     size_t nc = p->n / 2;
     long cs = chunk_signals(nc, count);
+    if (p->ct4) {
+        /* column pass, then the row pass with the Kiss post-pass fused */
+        void *w = fafi_workspace(p, (size_t)cs * nc * 2 * sizeof(float));
+        if (!w) return -1;
+        const long fo = odist * (inter ? 2 : 1);
+        for (long b0 = 0; b0 < count; b0 += cs) {
+            long c = count - b0 < cs ? count - b0 : cs;
+            int rc = fafk_ct4(FAFK_F4_IN_PAIR, inter ? FAFK_F4_POST_INTER : FAFK_F4_POST_SPLIT, 0,
+                              &p->a, &p->b, p->twn, NULL, p->post, x + b0 * idist, NULL, idist,
+                              w, (long)nc, orr + b0 * fo, inter ? NULL : oi + b0 * fo, fo, c,
+                              p->scale, stream);
+            if (kcerr(rc)) return -1;
+        }
+        return 0;
+    }
     float *ws = fafi_workspace(p, (size_t)cs * nc * 4 * sizeof(float));
     if (!ws) return -1;
     float *tr = ws, *ti = ws + (size_t)cs * nc;
@@ -147,6 +177,21 @@ int fafi_run_c2r(fafh_plan *p, const float *ir, const float *ii, long idist, flo
     if (p->kind != PK_C2R_4S) { fafi_err("plan is not a C2R transform"); return -1; }
     size_t nc = p->n / 2;
     long cs = chunk_signals(nc, count);
+    if (p->ct4) {
+        /* Kiss pre-pass fused into the column pass, real pairs out of the row pass */
+        void *w = fafi_workspace(p, (size_t)cs * nc * 2 * sizeof(float));
+        if (!w) return -1;
+        const long fi = idist * (inter ? 2 : 1);
+        for (long b0 = 0; b0 < count; b0 += cs) {
+            long c = count - b0 < cs ? count - b0 : cs;
+            int rc = fafk_ct4(inter ? FAFK_F4_PRE_INTER : FAFK_F4_PRE_SPLIT, FAFK_F4_OUT_PAIR, 1,
+                              &p->a, &p->b, p->twn, p->pre, NULL, ir + b0 * fi,
+                              inter ? NULL : ii + b0 * fi, fi, w, (long)nc, y + b0 * odist, NULL,
+                              odist, c, p->scale, stream);
+            if (kcerr(rc)) return -1;
+        }
+        return 0;
+    }
     float *ws = fafi_workspace(p, (size_t)cs * nc * 4 * sizeof(float));
     if (!ws) return -1;
     float *fr = ws, *fi = ws + (size_t)cs * nc;

@@ -516,6 +516,7 @@ static int plan_4step(fafh_plan *p, size_t n, int inv) {
         r >>= 1;
     p->cols = c;
     p->rows = r;
+    p->ct4 = fafi_ct_enabled() && fafk_ct4_supported(n1, n2);
     return 0;
 }
 
