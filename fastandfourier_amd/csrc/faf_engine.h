@@ -544,9 +544,10 @@ struct GFrameIn {            /* windowed frame packed as z[i] = (w x)[2i] + i (w
     static constexpr bool is_lds = false;
     FAF_DEV float2 ld(int i) const {
         const long a = 2L * i, b = a + 1;
+        if (b < avail)            /* both samples in the record: paired loads */
+            return make_float2(x[a] * w[a], x[b] * w[b]);
         float u = a < avail ? x[a] * w[a] : 0.0f;
-        float v = b < avail ? x[b] * w[b] : 0.0f;
-        return make_float2(u, v);
+        return make_float2(u, 0.0f);
     }
 };
 

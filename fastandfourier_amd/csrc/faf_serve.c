@@ -38,7 +38,6 @@
 #define SV_MAX_PAGES 4096
 #define SV_MAX_SLOTS 8
 #define SV_CWT_ROWS_PER_UNIT 4
-#define SV_BLOCK_MAX 1024
 #define SV_BLOCK_SMALL 512
 #define SV_ALIGN 256
 

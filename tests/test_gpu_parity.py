@@ -226,6 +226,38 @@ def test_rfft_irfft(n):
         G.destroy(ti)
 
 
+@pytest.mark.parametrize("n", [1024, 4096, 65536])
+@pytest.mark.parametrize("norm", ["none", "ortho", "forward"])
+def test_rfft_interleaved_layout_and_norms(n, norm):
+    """INTERLEAVED spectra (nb complex64) and every norm, including the
+    specialised four-step at 65536 whose Kiss passes are fused."""
+    say(f"case: rfft interleaved n={n} norm={norm}")
+    lib = G.L()
+    rng = np.random.default_rng(n + 7)
+    cnt, nb = 3, n // 2 + 1
+    x = rng.standard_normal((cnt, n)).astype(np.float32)
+    t = G.create("rfft", n, layout="interleaved", norm=norm)
+    ti = G.create("rfft", n, layout="interleaved", norm=norm, dir="inverse")
+    try:
+        dx = G.dev(x)
+        X = G.empty((cnt, 2 * nb))
+        y = G.empty((cnt, n))
+        torch.cuda.synchronize()
+        bx = fa.buffer_real(dx.data_ptr(), n)
+        bX = fa.buffer_interleaved(X.data_ptr(), nb)
+        by = fa.buffer_real(y.data_ptr(), n)
+        fa.check(lib.faf_execute_batch(t, C.byref(bX), C.byref(bx), cnt, 0, 0), "rfft inter")
+        fa.check(lib.faf_execute_batch(ti, C.byref(by), C.byref(bX), cnt, 0, 0), "irfft inter")
+        Xh = G.host(X).view(np.complex64)
+        fwd_s = {"none": 1.0, "ortho": 1 / np.sqrt(n), "forward": 1 / n}[norm]
+        truth = np.stack([O.fft64(r.astype(np.float64))[0][:nb] for r in x]) * fwd_s
+        assert rel_l2(Xh, truth) < tol(n)
+        assert rel_l2(G.host(y), x) < 2 * tol(n)
+    finally:
+        G.destroy(t)
+        G.destroy(ti)
+
+
 def test_rfft_golden_reference_outputs(golden):
     for n in (1024, 4096, 16384):
         say(f"case: golden rfft n={n}")
