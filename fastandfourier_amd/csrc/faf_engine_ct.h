@@ -158,6 +158,18 @@ void kc_r2c(fafd_fft P, const float2 *post, const float *x, long in_dist, float 
     GPairIn src{x + sig * in_dist};
     typename Ct<false, NC>::Chain ch;
     ch.load_tw(lt, active, (const float2 *)P.tw);
+    /* the post-pass super-twiddles too are loaded up front, except at
+     * NC = 8192, where the extra registers would cost a wave per SIMD
+     * (measured: 4.1 -> 2.95 TB/s); elsewhere LDS, not registers, bounds
+     * the occupancy */
+    constexpr int NP = (NC / 2 + T) / T;
+    constexpr bool HOIST = NC != 8192;
+    float2 pw[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        const int k = lt + i * T;
+        if (HOIST && active && k > 0 && k <= NC / 2) pw[i] = post[k];
+    }
     ch.run(src, lds, lds, lt, active);
     __syncthreads();
     if (!active) return;
@@ -165,7 +177,7 @@ void kc_r2c(fafd_fft P, const float2 *post, const float *x, long in_dist, float 
     float *oim = out_im + sig * out_dist;
     float2 *oi = (float2 *)out_re + sig * out_dist;
 #pragma unroll
-    for (int i = 0; i < (NC / 2 + T) / T; ++i) {
+    for (int i = 0; i < NP; ++i) {
         const int k = lt + i * T;
         if (k > NC / 2) break;
         const int m = NC - k;
@@ -175,7 +187,7 @@ void kc_r2c(fafd_fft P, const float2 *post, const float *x, long in_dist, float 
             xk = make_float2(z.x + z.y, 0.0f);
             xm = make_float2(z.x - z.y, 0.0f);
         } else {
-            kiss_post(lds.ld(k), lds.ld(m), post[k], xk, xm);
+            kiss_post(lds.ld(k), lds.ld(m), HOIST ? pw[i] : post[k], xk, xm);
         }
         if (INTER_OUT) {
             oi[k] = cscale(xk, scale);
@@ -285,13 +297,20 @@ void kc_stft(fafd_fft P, const float2 *post, const float *win, const float *x, l
     GFrameIn src{x + rec * rec_dist + f * hop, win, rec_len - f * hop};
     typename Ct<false, NC>::Chain ch;
     ch.load_tw(lt, active, (const float2 *)P.tw);
+    constexpr int NP = (NC / 2 + T) / T;
+    float2 pw[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        const int k = lt + i * T;
+        if (active && k > 0 && k <= NC / 2) pw[i] = post[k];
+    }
     ch.run(src, lds, lds, lt, active);
     __syncthreads();
     if (!active) return;
     float *ore = out_re + rec * rec_out_dist + f * frame_dist;
     float *oim = out_im + rec * rec_out_dist + f * frame_dist;
 #pragma unroll
-    for (int i = 0; i < (NC / 2 + T) / T; ++i) {
+    for (int i = 0; i < NP; ++i) {
         const int k = lt + i * T;
         if (k > NC / 2) break;
         const int m = NC - k;
@@ -301,7 +320,7 @@ void kc_stft(fafd_fft P, const float2 *post, const float *win, const float *x, l
             xk = make_float2(z.x + z.y, 0.0f);
             xm = make_float2(z.x - z.y, 0.0f);
         } else {
-            kiss_post(lds.ld(k), lds.ld(m), post[k], xk, xm);
+            kiss_post(lds.ld(k), lds.ld(m), pw[i], xk, xm);
         }
         ore[k] = xk.x; oim[k] = xk.y;
         ore[m] = xm.x; oim[m] = xm.y;
