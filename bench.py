@@ -36,8 +36,10 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # the board ramps its clocks over the first ~50 launches (about 25 ms of
+    # this workload); 100 warm-up launches put the timed steps at steady state
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--only", default="", help="comma list of extra workloads")
     ap.add_argument("--no-cpu", action="store_true",
@@ -172,7 +174,7 @@ def main():
 
     # copy ceiling on this box (read + write)
     z = torch.empty_like(x)
-    ms_copy = timed(lambda: z.copy_(x), 10, 2)
+    ms_copy = timed(lambda: z.copy_(x), 20, 5)
     extra["d2d_copy_gbs"] = round(2 * x.numel() * 4 / (ms_copy * 1e-3) / 1e9, 1)
     del z
     lib.faf_destroy_transform(t)
@@ -244,8 +246,8 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
             bo = fa.buffer_hermitian(re.data_ptr(), im.data_ptr(), nb)
             by = fa.buffer_real(y.data_ptr(), n)
             ab = b * (4 * n + 8 * nb)
-            ms = timed(lambda: fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0)), 5, 1)
-            ms2 = timed(lambda: fa.check(lib.faf_execute_batch(ti, C.byref(by), C.byref(bo), b, 0, 0)), 5, 1)
+            ms = timed(lambda: fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0)), 20, 10)
+            ms2 = timed(lambda: fa.check(lib.faf_execute_batch(ti, C.byref(by), C.byref(bo), b, 0, 0)), 20, 10)
             extra[f"r2c_{n}_gbs"] = round(ab / (ms * 1e-3) / 1e9, 1)
             extra[f"c2r_{n}_gbs"] = round(ab / (ms2 * 1e-3) / 1e9, 1)
             lib.faf_destroy_transform(t)
@@ -268,7 +270,7 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
         y = torch.empty_like(x)
         bi = fa.buffer_real(x.data_ptr(), n)
         bo = fa.buffer_real(y.data_ptr(), n)
-        ms = timed(lambda: fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0)), 5, 1)
+        ms = timed(lambda: fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0)), 20, 10)
         extra["notch_pipeline_4096_gbs"] = round(b * 8 * n / (ms * 1e-3) / 1e9, 1)
         lib.faf_destroy_transform(t)
         del x, y
@@ -283,7 +285,7 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
         t = lib.faf_create_fft(C.byref(fa.config_init(n)))
         bi = fa.buffer_split(xr.data_ptr(), xi.data_ptr(), n)
         bo = fa.buffer_split(yr.data_ptr(), yi.data_ptr(), n)
-        ms = timed(lambda: fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0)), 50, 5)
+        ms = timed(lambda: fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0)), 200, 50)
         extra["c1_c2c_4096x256_us"] = round(ms * 1e3, 2)
         extra["c1_c2c_4096x256_gbs"] = round(b * 16 * n / (ms * 1e-3) / 1e9, 1)
         lib.faf_destroy_transform(t)
@@ -299,7 +301,7 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
         xr = torch.rand(tot, device="cuda") * 2 - 1
         xi = torch.rand(tot, device="cuda") * 2 - 1
         yr, yi = torch.empty_like(xr), torch.empty_like(xi)
-        ms = timed(lambda: fa.check(lib.faf_ragged_execute(r, yr.data_ptr(), yi.data_ptr(), xr.data_ptr(), xi.data_ptr())), 10, 2)
+        ms = timed(lambda: fa.check(lib.faf_ragged_execute(r, yr.data_ptr(), yi.data_ptr(), xr.data_ptr(), xi.data_ptr())), 50, 20)
         extra["c3_ragged_20000_ms"] = round(ms, 3)
         extra["c3_ragged_gbs"] = round(16 * tot / (ms * 1e-3) / 1e9, 1)
         lib.faf_ragged_destroy(r)
@@ -314,7 +316,7 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
         x = torch.randn((ch, L), device="cuda")
         re = torch.empty((ch, nf, nb), device="cuda")
         im = torch.empty((ch, nf, nb), device="cuda")
-        ms = timed(lambda: fa.check(lib.faf_execute_stft(t, re.data_ptr(), im.data_ptr(), x.data_ptr(), L, ch, 0)), 3, 1)
+        ms = timed(lambda: fa.check(lib.faf_execute_stft(t, re.data_ptr(), im.data_ptr(), x.data_ptr(), L, ch, 0)), 10, 3)
         extra["stft_64x2^22_ms"] = round(ms, 3)
         extra["stft_gbs"] = round((4 * L * ch + 8 * nb * nf * ch) / (ms * 1e-3) / 1e9, 1)
         lib.faf_destroy_transform(t)
@@ -361,7 +363,7 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
         segs = 1024
         x = torch.randn((segs, n), device="cuda")
         z = torch.empty((segs, rows, n, 2), device="cuda")
-        ms = timed(lambda: fa.check(lib.faf_execute_cwt_complex(t, z.data_ptr(), x.data_ptr(), segs, 0)), 3, 1)
+        ms = timed(lambda: fa.check(lib.faf_execute_cwt_complex(t, z.data_ptr(), x.data_ptr(), segs, 0)), 20, 5)
         extra["cwt_1024seg_ms"] = round(ms, 3)
         extra["cwt_gbs"] = round(segs * (4 * n + 8 * n * rows) / (ms * 1e-3) / 1e9, 1)
         lib.faf_destroy_transform(t)
