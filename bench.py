@@ -285,9 +285,21 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
         t = lib.faf_create_fft(C.byref(fa.config_init(n)))
         bi = fa.buffer_split(xr.data_ptr(), xi.data_ptr(), n)
         bo = fa.buffer_split(yr.data_ptr(), yi.data_ptr(), n)
+        ti = lib.faf_create_inverse(t)
+        zr, zi = torch.empty_like(xr), torch.empty_like(xi)
+        bz = fa.buffer_split(zr.data_ptr(), zi.data_ptr(), n)
         ms = timed(lambda: fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0)), 200, 50)
+        ms_fi = timed(lambda: (fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0)),
+                               fa.check(lib.faf_execute_batch(ti, C.byref(bz), C.byref(bo), b, 0, 0))),
+                      200, 50)
+        torch.cuda.synchronize()
+        rt = (torch.linalg.vector_norm(torch.complex(zr, zi) - torch.complex(xr, xi)) /
+              torch.linalg.vector_norm(torch.complex(xr, xi))).item()
         extra["c1_c2c_4096x256_us"] = round(ms * 1e3, 2)
         extra["c1_c2c_4096x256_gbs"] = round(b * 16 * n / (ms * 1e-3) / 1e9, 1)
+        extra["c1_fwd_inv_4096x256_us"] = round(ms_fi * 1e3, 2)
+        extra["c1_round_trip_rel_l2"] = rt
+        lib.faf_destroy_transform(ti)
         lib.faf_destroy_transform(t)
 
     # C3 ragged: 20,000 signals, one persistent launch
@@ -452,6 +464,13 @@ def run_serving(lib, fa, torch, np, extra, count=100000, rate=200000.0):
         log("c5 served at", rate)
         main = replay(arr, count)
         main["offered"] = rate
+        # the same stream with a 30 us batching window (trades batch size
+        # for latency)
+        fa.check(lib.faf_server_set_window(s, 30.0), "window")
+        w30 = replay(arr, count)
+        w30["offered"] = rate
+        fa.check(lib.faf_server_set_window(s, 60.0), "window")
+        extra["c5_serving_window30"] = w30
         sweep = {}
         for r in (400000.0, 800000.0, 1600000.0):
             log("c5 sweep", r)

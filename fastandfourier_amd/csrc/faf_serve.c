@@ -147,6 +147,7 @@ struct faf_server {
     slot slots[SV_MAX_SLOTS];
     int nslots;
     _Atomic int plain;       /* one launch + synchronize per request */
+    _Atomic double window_ns;
     uint64_t stamp;
     /* batch scratch */
     sentry **bk_e;
@@ -958,7 +959,7 @@ static int maybe_launch(faf_server *s) {
     } else {
         if (busy) {
             uint64_t age = now_ns() - rec(s, ring_at(&s->ready, 0))->t_arr;
-            if ((double)age < s->cfg.window_us * 1e3 && s->ready.n < (size_t)s->cfg.max_batch)
+            if ((double)age < atomic_load(&s->window_ns) && s->ready.n < (size_t)s->cfg.max_batch)
                 return 0;
         }
         take = s->ready.n < (size_t)s->cfg.max_batch ? (int)s->ready.n : s->cfg.max_batch;
@@ -1119,6 +1120,7 @@ faf_server *faf_server_create(const faf_server_config *cfg) {
     s->nfr = 1;
     s->nslots = c.streams;
     atomic_store(&s->plain, c.plain ? 1 : 0);
+    atomic_store(&s->window_ns, c.window_us * 1e3);
     for (int i = 0; i < s->nslots; ++i) {
         slot *sl = &s->slots[i];
         sl->stream = fafk_stream_create();
@@ -1137,6 +1139,13 @@ faf_server *faf_server_create(const faf_server_config *cfg) {
     }
     s->started = 1;
     return s;
+}
+
+int faf_server_set_window(faf_server *s, double window_us) {
+    if (!s || !(window_us >= 0.0)) { fafi_err("serve: bad window"); return -1; }
+    faf_server_drain(s);
+    atomic_store(&s->window_ns, window_us * 1e3);
+    return 0;
 }
 
 int faf_server_set_plain(faf_server *s, int on) {

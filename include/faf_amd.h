@@ -149,6 +149,8 @@ faf_server *faf_server_create(const faf_server_config *cfg);
 void faf_server_destroy(faf_server *s);     /* waits for work in flight */
 /* switch plain per-request service on or off (waits for work in flight) */
 int faf_server_set_plain(faf_server *s, int on);
+/* change the batching window (waits for work in flight) */
+int faf_server_set_window(faf_server *s, double window_us);
 /* build and upload the plan of (kind, n) now rather than at first use */
 int faf_server_prepare(faf_server *s, faf_req_kind kind, size_t n);
 /* bytes of the result of one (kind, n) request, 0 if not servable */
