@@ -299,6 +299,33 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
         extra["c1_c2c_4096x256_gbs"] = round(b * 16 * n / (ms * 1e-3) / 1e9, 1)
         extra["c1_fwd_inv_4096x256_us"] = round(ms_fi * 1e3, 2)
         extra["c1_round_trip_rel_l2"] = rt
+
+        # the same forward+inverse captured once into a HIP graph and replayed
+        fa.check(lib.faf_graph_begin(), "graph begin")
+        rcs = (lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0),
+               lib.faf_execute_batch(ti, C.byref(bz), C.byref(bo), b, 0, 0))
+        g = lib.faf_graph_end()
+        if rcs != (0, 0) or not g:
+            raise RuntimeError("graph capture: " + lib.faf_get_error().decode())
+        ms_g = timed(lambda: fa.check(lib.faf_graph_launch(g), "graph"), 200, 50)
+
+        def latency(fn, reps=300):
+            """host wall time of one call + synchronize (what a caller waits)"""
+            for _ in range(20):
+                fn()
+                torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+                torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / reps * 1e6
+
+        lat_e = latency(lambda: (lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0),
+                                 lib.faf_execute_batch(ti, C.byref(bz), C.byref(bo), b, 0, 0)))
+        lat_g = latency(lambda: lib.faf_graph_launch(g))
+        extra["c1_graph_fwd_inv_4096x256_us"] = round(ms_g * 1e3, 2)
+        extra["c1_fwd_inv_call_latency_us"] = {"eager": round(lat_e, 1), "graph": round(lat_g, 1)}
+        lib.faf_graph_destroy(g)
         lib.faf_destroy_transform(ti)
         lib.faf_destroy_transform(t)
 

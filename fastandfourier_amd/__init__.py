@@ -164,6 +164,9 @@ _SIGS = {
     "faf_amd_set_stream": (c_int, [c_void_p]), "faf_amd_get_stream": (c_void_p, []),
     "faf_amd_set_sync": (c_int, [c_int]), "faf_amd_synchronize": (c_int, []),
     "faf_amd_device_ready": (c_int, []),
+    "faf_graph_begin": (c_int, []), "faf_graph_end": (c_void_p, []),
+    "faf_graph_launch": (c_int, [c_void_p]), "faf_graph_nodes": (c_int, [c_void_p]),
+    "faf_graph_destroy": (None, [c_void_p]),
     "faf_execute_batch": (c_int, [PT, PB, PB, c_size_t, c_size_t, c_size_t]),
     "faf_ragged_create": (c_void_p, [POINTER(c_size_t), c_size_t, c_uint32]),
     "faf_ragged_total": (c_size_t, [c_void_p]),

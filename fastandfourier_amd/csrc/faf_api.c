@@ -85,7 +85,97 @@ int faf_amd_set_specialised(int on) { t_no_ct = !on; return 0; }
 int fafi_ct_enabled(void) { return !t_no_ct; }
 float faf_amd_last_kernel_ms(void) { return t_last_ms; }
 
+/* ---- graph capture of executes (thread-local) ---- */
+static _Thread_local int t_capture;        /* 1 while faf_graph_begin is open */
+static _Thread_local int t_cap_failed;     /* an execute failed inside the capture */
+static _Thread_local void *t_cap_prev;     /* caller's stream before the capture */
+static _Thread_local int t_cap_prev_async;
+static _Thread_local void *t_cap_own;      /* stream made for a capture on the default stream */
+
+struct faf_graph {
+    void *exec;
+    int nodes;
+};
+
+int fafi_capturing(void) { return t_capture; }
+
+/* create-time entry: plans upload their tables synchronously, which a
+ * capture cannot hold */
+int fafi_create_ready(void) {
+    if (t_capture) {
+        fafi_err("plans cannot be created while a graph capture is open");
+        return -1;
+    }
+    return fafi_runtime_ready();
+}
+
+int faf_graph_begin(void) {
+    if (fafi_runtime_ready()) return -1;
+    if (t_capture) { fafi_err("faf_graph_begin: a capture is already open"); return -1; }
+    void *s = t_stream;
+    if (!s) {
+        /* the legacy default stream cannot be captured; capture on a private one */
+        t_cap_own = fafk_stream_create();
+        if (!t_cap_own) { fafi_err("faf_graph_begin: no stream"); return -1; }
+        s = t_cap_own;
+    }
+    if (fafk_capture_begin(s)) {
+        fafi_err("%s", fafk_last_error());
+        fafk_stream_destroy(t_cap_own);
+        t_cap_own = NULL;
+        return -1;
+    }
+    t_cap_prev = t_stream;
+    t_cap_prev_async = t_async;
+    t_stream = s;
+    t_async = 1;
+    t_capture = 1;
+    t_cap_failed = 0;
+    return 0;
+}
+
+faf_graph *faf_graph_end(void) {
+    if (!t_capture) { fafi_err("faf_graph_end: no capture is open"); return NULL; }
+    int nodes = 0;
+    void *x = fafk_capture_end(t_stream, &nodes);
+    t_stream = t_cap_prev;
+    t_async = t_cap_prev_async;
+    t_capture = 0;
+    fafk_stream_destroy(t_cap_own);
+    t_cap_own = NULL;
+    if (!x) { fafi_err("faf_graph_end: %s", fafk_last_error()); return NULL; }
+    if (t_cap_failed) {
+        fafk_graph_destroy(x);
+        fafi_err("faf_graph_end: an execute failed inside the capture");
+        return NULL;
+    }
+    faf_graph *g = calloc(1, sizeof *g);
+    if (!g) { fafk_graph_destroy(x); fafi_err("faf_graph_end: out of memory"); return NULL; }
+    g->exec = x;
+    g->nodes = nodes;
+    return g;
+}
+
+int faf_graph_launch(const faf_graph *g) {
+    if (!g) { fafi_err("faf_graph_launch: null graph"); return -1; }
+    if (t_capture) { fafi_err("faf_graph_launch: inside an open capture"); return -1; }
+    if (fafk_graph_launch(g->exec, t_stream) || (!t_async && fafk_sync(t_stream))) {
+        fafi_err("%s", fafk_last_error());
+        return -1;
+    }
+    return 0;
+}
+
+int faf_graph_nodes(const faf_graph *g) { return g ? g->nodes : 0; }
+
+void faf_graph_destroy(faf_graph *g) {
+    if (!g) return;
+    fafk_graph_destroy(g->exec);
+    free(g);
+}
+
 int faf_amd_synchronize(void) {
+    if (t_capture) { fafi_err("faf_amd_synchronize: a graph capture is open"); return -1; }
     if (fafi_runtime_ready()) return -1;
     if (fafk_sync(t_stream)) { fafi_err("%s", fafk_last_error()); return -1; }
     return 0;
@@ -257,7 +347,7 @@ faf_transform *faf_create_fft(const faf_config *cfg) {
         return NULL;
     }
     if (check_precision(c.precision, rader ? "Rader" : blue ? "Bluestein" : "FFT")) return NULL;
-    if (fafi_runtime_ready()) return NULL;
+    if (fafi_create_ready()) return NULL;
     faf_transform *t = new_transform();
     if (!t) return NULL;
     int inv = c.dir == FAF_DIR_INVERSE;
@@ -305,7 +395,7 @@ faf_transform *faf_create_rfft(const faf_config *cfg) {
         fafi_err("R2C size 2 has a one-point inner transform; use n >= 4");
         return NULL;
     }
-    if (fafi_runtime_ready()) return NULL;
+    if (fafi_create_ready()) return NULL;
     faf_transform *t = new_transform();
     if (!t) return NULL;
     int inv = c.dir == FAF_DIR_INVERSE;
@@ -336,7 +426,7 @@ faf_transform *faf_create_stft(const faf_config *cfg) {
     }
     if (c.layout == FAF_LAYOUT_REAL) {
         if (check_precision(c.precision, "STFT")) return NULL;
-        if (fafi_runtime_ready()) return NULL;
+        if (fafi_create_ready()) return NULL;
         faf_transform *t = new_transform();
         if (!t) return NULL;
         c.dir = FAF_DIR_FORWARD;
@@ -433,7 +523,7 @@ void faf_destroy_transform(faf_transform *t) {
     if (!t) return;
     if (t->amd) {
         fafh_plan *p = t->amd;
-        if (p->kind != PK_NONE) fafk_sync(fafi_stream());
+        if (p->kind != PK_NONE && !t_capture) fafk_sync(fafi_stream());
         fafi_plan_free(p);
     }
     free(t->code);
@@ -470,6 +560,10 @@ static int planes_stage_in(fafh_plan *p, planes_t *ps, void *stream) {
         if (host[i]) need += (ps->bytes[i] + 255) & ~(size_t)255;
     }
     if (!need) return 0;
+    if (t_capture) {
+        fafi_err("host buffers cannot be captured into a graph; pass device pointers");
+        return -1;
+    }
     char *base = fafi_staging(p, need);
     if (!base) return -1;
     size_t off = 0;
@@ -508,9 +602,12 @@ static int exec_planes(const faf_transform *t, planes_t *ps, long count, long id
     void *stream = fafi_stream();
     fafh_plan *p = t->amd;
     g_err[0] = '\0';
-    if (planes_stage_in(p, ps, stream)) return -1;
+    if (planes_stage_in(p, ps, stream)) {
+        t_cap_failed |= t_capture;
+        return -1;
+    }
     void *ev0 = NULL, *ev1 = NULL;
-    if (t_timing) {
+    if (t_timing && !t_capture) {
         ev0 = fafk_event_create();
         ev1 = fafk_event_create();
         if (ev0) fafk_event_record(ev0, stream);
@@ -527,6 +624,7 @@ static int exec_planes(const faf_transform *t, planes_t *ps, long count, long id
     fafk_event_destroy(ev0);
     fafk_event_destroy(ev1);
     if (rc && !g_err[0]) fafi_err("%s", fafk_last_error());
+    if (rc) t_cap_failed |= t_capture;
     return rc;
 }
 

@@ -237,6 +237,10 @@ int fafi_pipe_refresh(fafh_plan *p, void *stream) {
         p->ops.h_re = p->h_host_re;
         p->ops.h_im = p->h_host_im;
     } else {
+        if (fafi_capturing()) {
+            fafi_err("a graph capture needs the bound spectrum in device memory");
+            return -1;
+        }
         if (p->h_dev_cap < cap) {
             fafk_free(p->h_dev);
             p->h_dev = fafk_malloc(cap);

@@ -596,7 +596,7 @@ int fafi_run_icwt(fafh_plan *p, const float *w, long idist, float *y, long odist
 
 static faf_transform *cwt_transform(const faf_cwt_config *cfg, int inverse_kind) {
     if (!cfg) { fafi_err("cwt: null argument"); return NULL; }
-    if (fafi_runtime_ready()) return NULL;
+    if (fafi_create_ready()) return NULL;
     faf_transform *t = calloc(1, sizeof(faf_transform));
     fafh_plan *p = fafi_plan_new();
     if (!t || !p) { free(t); free(p); fafi_err("cwt: alloc failed"); return NULL; }

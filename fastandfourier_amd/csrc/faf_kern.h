@@ -31,6 +31,10 @@ int fafk_event_query(void *ev);
 int fafk_event_sync(void *ev);
 float fafk_event_ms(void *a, void *b);
 int fafk_device_info(int *cus, size_t *lds_per_block, size_t *total_mem);
+int fafk_capture_begin(void *stream);
+void *fafk_capture_end(void *stream, int *nodes);
+int fafk_graph_launch(void *exec, void *stream);
+void fafk_graph_destroy(void *exec);
 int fafk_fill_u32(unsigned *p, int n, unsigned v, void *stream);
 int fafk_window_inter(const float *x, long in_dist, const float *w, long n, float *y,
                       long count, void *stream);

@@ -170,6 +170,47 @@ extern "C" float fafk_event_ms(void *a, void *b) {
     return ms;
 }
 
+/* ---- stream capture into a replayable graph ----
+ * Relaxed mode: the host code itself refuses every synchronising or
+ * allocating call while a capture is open, so the runtime need not. */
+extern "C" int fafk_capture_begin(void *stream) {
+    hipError_t e = hipStreamBeginCapture(S(stream), hipStreamCaptureModeRelaxed);
+    return e == hipSuccess ? 0 : fail("hipStreamBeginCapture", e);
+}
+
+/* Ends the capture and instantiates it; NULL (error set) if the capture was
+ * invalidated or holds nothing.  *nodes receives the graph's node count. */
+extern "C" void *fafk_capture_end(void *stream, int *nodes) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(S(stream), &g);
+    if (e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        fail("hipStreamEndCapture", e != hipSuccess ? e : hipErrorStreamCaptureInvalidated);
+        return nullptr;
+    }
+    size_t n = 0;
+    e = hipGraphGetNodes(g, nullptr, &n);
+    if (e == hipSuccess && n == 0) e = hipErrorInvalidValue;
+    hipGraphExec_t x = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        fail(n ? "hipGraphInstantiate" : "empty capture", e);
+        return nullptr;
+    }
+    if (nodes) *nodes = (int)n;
+    return (void *)x;
+}
+
+extern "C" int fafk_graph_launch(void *x, void *stream) {
+    hipError_t e = hipGraphLaunch((hipGraphExec_t)x, S(stream));
+    return e == hipSuccess ? 0 : fail("hipGraphLaunch", e);
+}
+
+extern "C" void fafk_graph_destroy(void *x) {
+    if (x) (void)hipGraphExecDestroy((hipGraphExec_t)x);
+}
+
 extern "C" int fafk_device_info(int *cus, size_t *lds_per_block, size_t *total_mem) {
     hipDeviceProp_t p;
     hipError_t e = hipGetDeviceProperties(&p, 0);

@@ -380,6 +380,11 @@ void *fafi_workspace(fafh_plan *p, size_t bytes) {
         return NULL;
     }
     if (p->ws_bytes >= bytes) return p->ws;
+    if (fafi_capturing()) {
+        fafi_err("the workspace must grow to %zu bytes, which a graph capture cannot do; "
+                 "execute once before capturing", bytes);
+        return NULL;
+    }
     fafk_sync(fafi_stream());
     fafk_free(p->ws);
     p->ws = fafk_malloc(bytes);
@@ -390,6 +395,7 @@ void *fafi_workspace(fafh_plan *p, size_t bytes) {
 
 void *fafi_staging(fafh_plan *p, size_t bytes) {
     if (p->stg_bytes >= bytes) return p->stg;
+    if (fafi_capturing()) { fafi_err("staging cannot grow inside a graph capture"); return NULL; }
     fafk_sync(fafi_stream());
     fafk_free(p->stg);
     p->stg = fafk_malloc(bytes);

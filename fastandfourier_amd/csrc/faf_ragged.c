@@ -61,7 +61,7 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
         fafi_err("ragged: need 1..2^31-1 lengths");
         return NULL;
     }
-    if (fafi_runtime_ready()) return NULL;
+    if (fafi_create_ready()) return NULL;
     faf_ragged *r = calloc(1, sizeof *r);
     r->count = count;
     /* distinct lengths */

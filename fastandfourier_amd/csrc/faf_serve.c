@@ -1083,7 +1083,7 @@ void faf_server_destroy(faf_server *s) {
 }
 
 faf_server *faf_server_create(const faf_server_config *cfg) {
-    if (fafi_runtime_ready()) return NULL;
+    if (fafi_create_ready()) return NULL;
     faf_server_config c = cfg ? *cfg : faf_server_config_init();
     faf_server_config d = faf_server_config_init();
     if (!c.hbm_budget) c.hbm_budget = d.hbm_budget;

@@ -74,7 +74,7 @@ static float time_once(fafh_plan *p, void *x, void *y, long count, void *stream)
 
 int faf_amd_autotune(faf_transform *t, size_t count) {
     if (!t || !t->amd) { fafi_err("autotune: no transform"); return -1; }
-    if (fafi_runtime_ready()) return -1;
+    if (fafi_create_ready()) return -1;
     fafh_plan *p = t->amd;
     if (p->kind != PK_C2C && p->kind != PK_R2C && p->kind != PK_C2R && p->kind != PK_PIPE) {
         fafi_err("autotune: plan kind has a fixed launch shape");

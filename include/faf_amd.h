@@ -24,6 +24,27 @@ int faf_amd_set_sync(int on);
 int faf_amd_synchronize(void);
 int faf_amd_device_ready(void);              /* 0 if a gfx950 device is usable */
 
+/* ---- graph capture of executes (thread-local) ----
+ * Between faf_graph_begin and faf_graph_end, faf_execute* and
+ * faf_ragged_execute on this thread are recorded into a HIP graph instead of
+ * running; faf_graph_launch then replays the whole sequence with one launch
+ * on the thread's stream (synchronising when sync is on).  The capture uses
+ * the thread's stream, or a private stream when that is the default stream.
+ * While a capture is open:
+ *   - every buffer must be device memory (host staging cannot be captured);
+ *   - each plan must already have run once at its largest batch, so that its
+ *     workspace is allocated (an execute that would grow it fails);
+ *   - creating plans, autotuning and faf_amd_synchronize fail.
+ * An execute that fails inside the capture makes faf_graph_end return NULL.
+ * Replays read and write the buffers that were captured, so they see the
+ * buffers' contents at replay time.  Plans must outlive their graphs. */
+typedef struct faf_graph faf_graph;
+int faf_graph_begin(void);
+faf_graph *faf_graph_end(void);
+int faf_graph_launch(const faf_graph *g);
+int faf_graph_nodes(const faf_graph *g);     /* kernel and copy nodes in the graph */
+void faf_graph_destroy(faf_graph *g);
+
 /* ---- batched execute ----
  * Runs `count` independent signals of the plan in one launch.  `in` and
  * `out` describe signal 0 exactly as for faf_execute; signal i starts

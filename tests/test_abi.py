@@ -106,6 +106,12 @@ def test_creation_fails_loudly_without_a_device():
     t = L.faf_create_fft(C.byref(fa.config_init(64)))
     assert not t
     assert b"GPU runtime unavailable" in L.faf_get_error()
+    # graph capture needs the device too; the null-argument paths need nothing
+    assert L.faf_graph_begin() == -1
+    assert b"GPU runtime unavailable" in L.faf_get_error()
+    assert not L.faf_graph_end()
+    assert L.faf_graph_launch(None) == -1 and L.faf_graph_nodes(None) == 0
+    L.faf_graph_destroy(None)
 
 
 def test_server_config_defaults_and_no_silent_cpu_path():

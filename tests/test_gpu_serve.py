@@ -237,6 +237,10 @@ def test_serve_admission_waits_for_budget():
             y = fetch(s, t)
             assert rel_l2(y, O.fft64(xs[i][0].astype(np.complex128)).ravel()) <= tol(n)
             lib.faf_server_release(s, t)
+            # one release admits exactly one more request; waiting for it before the
+            # next release makes every later head-of-line request meet a full budget
+            if i + 3 < len(tks):
+                assert lib.faf_server_wait(s, tks[i + 3]) == 0
         st = fa.FafServerStats()
         lib.faf_server_stats_get(s, C.byref(st))
         assert st.admission_waits >= 7 and st.peak_hbm_bytes <= 3 * n * 8 + 4096
