@@ -1,0 +1,294 @@
+/*
+ * faf_engine_rs.h -- single-pass 32768-point complex FFT held in registers:
+ * the core of the N = 65536 rfft / irfft (Kiss packing halves it to 32768
+ * complex points) and of C2C 32768.
+ *
+ * One workgroup of 1024 threads owns a whole signal, 32 complex values per
+ * thread (64 VGPRs; 256 KB per signal, half the CU's register file), so the
+ * signal makes one HBM read and one HBM write.  The four-step split of the
+ * same length makes two of each.  32768 = 32 * 32 * 32, natural order in and
+ * out:
+ *   pass 1  thread t          DFT-32 over m of z[t + 1024 m], times W_32768^(t k1)
+ *   pass 2  thread (t1, k1)   DFT-32 over t2 of B[t1 + 32 t2][k1], times W_1024^(t1 k2a)
+ *   pass 3  thread (k1, k2a)  DFT-32 over t1, giving X[k1 + 32 k2a + 1024 k2b]
+ * so after pass 3 thread tid holds X[tid + 1024 j], j = 0..31, and stores of
+ * one register index are contiguous across the workgroup, as are the loads
+ * of pass 1.  Between passes the values go through LDS one plane (re, then
+ * im) at a time: 32 x 1056 floats = 132 KB.  The 1056-float row pitch puts
+ * lanes 32..63 of a wave 32 banks away from lanes 0..31, and the pass-2 to
+ * pass-3 layout uses a 33-float pitch between k1 rows, so every LDS access
+ * of a wave touches 64 distinct banks.
+ *
+ * Twiddles: a thread loads the five powers W^(s 2^i) of its twiddle step s
+ * from the FP64-derived W_32768 table and forms W^(s k) over the set bits of
+ * k (at most four products), as the staged kernels do.
+ */
+#ifndef FAF_ENGINE_RS_H
+#define FAF_ENGINE_RS_H
+#define FAF_ENGINE_NO_PLAIN_KERNELS
+#include "faf_engine.h"
+
+namespace rs {
+
+constexpr int NC = 32768;        /* complex points per signal */
+constexpr int T = 1024;          /* threads per signal */
+constexpr int P = 1056;          /* LDS row pitch in floats */
+constexpr int LDS_BYTES = 32 * P * 4;
+
+/* cos(pi j / 32), j = 0..16 */
+FAF_DEV constexpr float cq(int j) {
+    return j == 0 ? 1.0f
+         : j == 1 ? 0.995184726672196886245f
+         : j == 2 ? 0.980785280403230449126f
+         : j == 3 ? 0.956940335732208864936f
+         : j == 4 ? 0.923879532511286756128f
+         : j == 5 ? 0.881921264348355029713f
+         : j == 6 ? 0.831469612302545237079f
+         : j == 7 ? 0.773010453362736960811f
+         : j == 8 ? 0.707106781186547524401f
+         : j == 9 ? 0.634393284163645498215f
+         : j == 10 ? 0.555570233019602224743f
+         : j == 11 ? 0.471396736825997648556f
+         : j == 12 ? 0.382683432365089771728f
+         : j == 13 ? 0.290284677254462367636f
+         : j == 14 ? 0.195090322016128267848f
+         : j == 15 ? 0.0980171403295606019942f
+         : 0.0f;
+}
+
+/* exp(-+ 2 pi i k / 64): forward sign -, inverse + */
+template <bool INV> FAF_DEV constexpr float2 w64(int k) {
+    k &= 63;
+    float c, s;                                   /* cos, sin of pi k / 32 */
+    if (k <= 16) { c = cq(k); s = cq(16 - k); }
+    else if (k <= 32) { c = -cq(32 - k); s = cq(k - 16); }
+    else if (k <= 48) { c = -cq(k - 32); s = -cq(48 - k); }
+    else { c = cq(64 - k); s = -cq(k - 48); }
+    return make_float2(c, INV ? s : -s);
+}
+
+template <bool INV> FAF_DEV void dft32(float2 *v) {
+    float2 e[16], o[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) { e[k] = v[2 * k]; o[k] = v[2 * k + 1]; }
+    dft16<INV>(e);
+    dft16<INV>(o);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) o[k] = k == 8 ? rot<INV>(o[k]) : cmul(o[k], w64<INV>(2 * k));
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        v[k] = cadd(e[k], o[k]);
+        v[k + 16] = csub(e[k], o[k]);
+    }
+}
+
+/* the powers W^(s 2^i), i < 5, of the table's root */
+FAF_DEV void tw_bases(const float2 *tw, int s, float2 *b) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) b[i] = tw[(s << i) & (NC - 1)];
+}
+
+/* v[k] *= W^(s k) for k = 1..31, each power a product over the set bits of k */
+FAF_DEV void tw_apply(float2 *v, const float2 *b) {
+    float2 w[32];
+#pragma unroll
+    for (int k = 1; k < 32; ++k) {
+        const int i = __builtin_ctz(k);
+        w[k] = k == (1 << i) ? b[i] : cmul(w[k & (k - 1)], b[i]);
+        v[k] = cmul(v[k], w[k]);
+    }
+}
+
+/* Exchange through LDS: the thread writes v[r] at lds[wb + r WS] and then
+ * holds lds[rb + r RS], one plane at a time.  The caller barriers between
+ * the previous reads of lds and this call. */
+template <int WS, int RS> FAF_DEV void xchg(float2 *v, float *lds, int wb, int rb) {
+#pragma unroll
+    for (int r = 0; r < 32; ++r) lds[wb + r * WS] = v[r].x;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 32; ++r) v[r].x = lds[rb + r * RS];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 32; ++r) lds[wb + r * WS] = v[r].y;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 32; ++r) v[r].y = lds[rb + r * RS];
+}
+
+/* In: v[m] = z[tid + 1024 m].  Out: v[j] = Z[tid + 1024 j].  b1 holds the
+ * twiddle bases of step tid (loaded with the signal); those of pass 2 are
+ * loaded here, while pass 1's values go through LDS.  Ends with LDS reads
+ * outstanding in other waves: barrier before reusing lds. */
+template <bool INV>
+FAF_DEV void fft32768(float2 *v, float *lds, const float2 *tw, const float2 *b1, int tid) {
+    dft32<INV>(v);
+    tw_apply(v, b1);
+    float2 b2[5];
+    tw_bases(tw, 32 * (tid & 31), b2);
+    xchg<P, 32>(v, lds, tid, (tid >> 5) * P + (tid & 31));
+    dft32<INV>(v);
+    tw_apply(v, b2);
+    __syncthreads();
+    xchg<P, 1>(v, lds, (tid >> 5) * 33 + (tid & 31), (tid >> 5) * P + (tid & 31) * 33);
+    dft32<INV>(v);
+}
+
+}  // namespace rs
+
+/* ------------------------------------------------------------------ */
+/* kernels: persistent, one workgroup per CU, signals s = blockIdx.x,   */
+/* blockIdx.x + gridDim.x, ...                                          */
+/* ------------------------------------------------------------------ */
+
+/* rfft N = 65536: x real (8-byte aligned rows), out N/2+1 bins, split or
+ * interleaved.  post = forward Kiss table of 32768 (slots 1..16384). */
+template <bool INTER_OUT>
+__global__ __launch_bounds__(1024) void kr_r2c(const float2 *tw, const float2 *post,
+                                               const float *x, long in_dist, float *out_re,
+                                               float *out_im, long out_dist, long count,
+                                               float scale) {
+    using namespace rs;
+    float *lds = (float *)faf_smem;
+    float2 *q = faf_smem;                         /* 16 x 1024 float2 for the post-pass */
+    /* Kiss twiddle of bin k = tid + 1024 j is -i W_65536^k = pb * W_64^j */
+    const float2 pb0 = threadIdx.x ? post[threadIdx.x] : make_float2(0.0f, -1.0f);
+    for (long s = blockIdx.x; s < count; s += gridDim.x) {
+        /* the thread index and pb are re-read every signal so that the
+         * compiler does not keep every derived offset and twiddle alive (and
+         * spilled) across the whole loop */
+        int tid = (int)threadIdx.x;
+        float2 pb = pb0;
+        asm volatile("" : "+v"(tid), "+v"(pb.x), "+v"(pb.y));
+        const int pt = (T - tid) & (T - 1);
+        float2 v[32], b1[5];
+        const float2 *z = (const float2 *)(x + s * in_dist);
+#pragma unroll
+        for (int m = 0; m < 32; ++m) v[m] = z[tid + T * m];
+        tw_bases(tw, tid, b1);
+        __syncthreads();                          /* the previous signal is out of LDS */
+        fft32768<false>(v, lds, tw, b1, tid);
+        float *ore = out_re + s * out_dist, *oim = out_im + s * out_dist;
+        float2 *oi = (float2 *)out_re + s * out_dist;
+        auto put = [&](int k, float2 xk) {
+            if (INTER_OUT) oi[k] = cscale(xk, scale);
+            else { ore[k] = xk.x * scale; oim[k] = xk.y * scale; }
+        };
+        /* Bin pairs (k, 32768 - k) with k = tid + 1024 j, j < 16: the mirror
+         * value is thread pt's value 31 - j (thread 0: its own 32 - j), so the
+         * upper halves go through LDS and each thread finishes the pairs of its
+         * lower half, X[k] and X[32768 - k] from one Kiss butterfly. */
+        __syncthreads();
+#pragma unroll
+        for (int j = 16; j < 32; ++j) q[(j - 16) * T + tid] = v[j];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            float2 xk, xm;
+            if (tid == 0 && j == 0) {
+                put(0, make_float2(v[0].x + v[0].y, 0.0f));
+                put(NC, make_float2(v[0].x - v[0].y, 0.0f));
+                const float2 h = q[0];            /* Z[16384], its own mirror */
+                kiss_post(h, h, cmul(pb, w64<false>(16)), xk, xm);
+                put(NC / 2, xk);
+                continue;
+            }
+            kiss_post(v[j], q[tid ? (15 - j) * T + pt : (16 - j) * T],
+                      cmul(pb, w64<false>(j)), xk, xm);
+            put(tid + T * j, xk);
+            put(tid ? pt + T * (31 - j) : T * (32 - j), xm);
+            if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+/* irfft N = 65536: in N/2+1 bins (split or interleaved), y real (8-byte
+ * aligned rows).  pre = inverse Kiss table of 32768; tw = inverse W_32768. */
+template <bool INTER_IN>
+__global__ __launch_bounds__(1024) void kr_c2r(const float2 *tw, const float2 *pre,
+                                               const float *in_re, const float *in_im,
+                                               long in_dist, float *y, long out_dist,
+                                               long count, float scale) {
+    using namespace rs;
+    float *lds = (float *)faf_smem;
+    float2 *q = faf_smem;                         /* 16 x 1024 float2 for the pre-pass */
+    /* Kiss pre-twiddle of bin k = tid + 1024 j is i W_65536^-k = pb * conj(W_64^j) */
+    const float2 pb0 = threadIdx.x ? pre[threadIdx.x] : make_float2(0.0f, 1.0f);
+    for (long s = blockIdx.x; s < count; s += gridDim.x) {
+        int tid = (int)threadIdx.x;               /* re-read per signal, as in kr_r2c */
+        float2 pb = pb0;
+        asm volatile("" : "+v"(tid), "+v"(pb.x), "+v"(pb.y));
+        const int pt = (T - tid) & (T - 1);
+        float2 v[32], b1[5];
+        const float *ire = in_re + s * in_dist, *iim = in_im + s * in_dist;
+        const float2 *ii = (const float2 *)in_re + s * in_dist;
+        auto get = [&](int k) {
+            return INTER_IN ? ii[k] : make_float2(ire[k], iim[k]);
+        };
+        tw_bases(tw, tid, b1);
+        /* Bin pairs (k, 32768 - k) with k = tid + 1024 j, j < 16: one Kiss
+         * butterfly gives F[k], this thread's value j, and F[32768 - k], thread
+         * pt's value 31 - j (thread 0: its own 32 - j), which goes through LDS.
+         * Every bin is read from HBM once. */
+        __syncthreads();                          /* the previous signal is out of LDS */
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const float2 a = get(tid + T * j);
+            const float2 b = get(tid ? pt + T * (31 - j) : T * (32 - j));
+            float2 fm;
+            if (tid == 0 && j == 0) {
+                v[0] = make_float2(a.x + b.x, a.x - b.x);
+                const float2 h = get(NC / 2);     /* X[16384], its own mirror */
+                kiss_pre(h, h, cmul(pb, w64<true>(16)), q[0], fm);
+                continue;
+            }
+            kiss_pre(a, b, cmul(pb, w64<true>(j)), v[j], fm);
+            q[tid ? (15 - j) * T + pt : (16 - j) * T] = fm;
+            if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 16; j < 32; ++j) v[j] = q[(j - 16) * T + tid];
+        __syncthreads();
+        fft32768<true>(v, lds, tw, b1, tid);
+        float2 *yo = (float2 *)(y + s * out_dist);
+#pragma unroll
+        for (int j = 0; j < 32; ++j) yo[tid + T * j] = cscale(v[j], scale);
+    }
+}
+
+/* C2C 32768, split or interleaved in and out (interleaved dists in complex
+ * elements, split dists in floats per plane) */
+template <bool INV, bool INTER_IN, bool INTER_OUT>
+__global__ __launch_bounds__(1024) void kr_c2c(const float2 *tw, const float *in_re,
+                                               const float *in_im, long in_dist,
+                                               float *out_re, float *out_im, long out_dist,
+                                               long count, float scale) {
+    using namespace rs;
+    float *lds = (float *)faf_smem;
+    for (long s = blockIdx.x; s < count; s += gridDim.x) {
+        int tid = (int)threadIdx.x;               /* re-read per signal, as in kr_r2c */
+        asm volatile("" : "+v"(tid));
+        float2 v[32], b1[5];
+        const float *ire = in_re + s * in_dist, *iim = in_im + s * in_dist;
+        const float2 *ii = (const float2 *)in_re + s * in_dist;
+#pragma unroll
+        for (int m = 0; m < 32; ++m) {
+            const int k = tid + T * m;
+            v[m] = INTER_IN ? ii[k] : make_float2(ire[k], iim[k]);
+        }
+        tw_bases(tw, tid, b1);
+        __syncthreads();
+        fft32768<INV>(v, lds, tw, b1, tid);
+        float *ore = out_re + s * out_dist, *oim = out_im + s * out_dist;
+        float2 *oi = (float2 *)out_re + s * out_dist;
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            const int k = tid + T * j;
+            if (INTER_OUT) oi[k] = cscale(v[j], scale);
+            else { ore[k] = v[j].x * scale; oim[k] = v[j].y * scale; }
+        }
+    }
+}
+
+#endif

@@ -65,6 +65,7 @@ typedef struct fafh_plan {
     int spb;
     int ct;                   /* use the compile-time specialised kernel */
     int ct4;                  /* four-step with specialised column/row passes */
+    int rs;                   /* single-pass 32768-point kernel (register-resident) */
     float scale;
     int interleaved;          /* buffers are interleaved complex */
     void *ws;                 /* device workspace */

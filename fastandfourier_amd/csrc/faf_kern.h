@@ -31,6 +31,20 @@ int fafk_event_query(void *ev);
 int fafk_event_sync(void *ev);
 float fafk_event_ms(void *a, void *b);
 int fafk_device_info(int *cus, size_t *lds_per_block, size_t *total_mem);
+/* single-pass 32768-point kernels (faf_kernels_rs.hip); grid 0 = one
+ * workgroup per CU */
+const char *fafk_rs_last_error(void);
+int fafk_rs_init(void);
+int fafk_rs_pair_ok(const void *p, long dist);
+int fafk_rs_r2c(int inter_out, const void *tw, const void *post, const float *x, long in_dist,
+                float *out_re, float *out_im, long out_dist, long count, float scale, int grid,
+                void *stream);
+int fafk_rs_c2r(int inter_in, const void *tw, const void *pre, const float *in_re,
+                const float *in_im, long in_dist, float *y, long out_dist, long count,
+                float scale, int grid, void *stream);
+int fafk_rs_c2c(int inv, int inter, const void *tw, const float *in_re, const float *in_im,
+                long in_dist, float *out_re, float *out_im, long out_dist, long count,
+                float scale, int grid, void *stream);
 int fafk_capture_begin(void *stream);
 void *fafk_capture_end(void *stream, int *nodes);
 int fafk_graph_launch(void *exec, void *stream);

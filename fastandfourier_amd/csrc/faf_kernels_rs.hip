@@ -1,0 +1,103 @@
+/*
+ * faf_kernels_rs.hip -- C-ABI launchers of the single-pass 32768-point
+ * kernels (faf_engine_rs.h): rfft / irfft N = 65536 and C2C 32768.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include "faf_engine_rs.h"
+#include "faf_kern.h"
+
+static thread_local char g_rs_err[160];
+
+static int rs_fail(const char *what, hipError_t e) {
+    snprintf(g_rs_err, sizeof g_rs_err, "%s: %s", what, hipGetErrorString(e));
+    return -1;
+}
+
+extern "C" const char *fafk_rs_last_error(void) { return g_rs_err; }
+
+static inline hipStream_t S(void *s) { return (hipStream_t)s; }
+
+template <class K> static void rs_attr(K k) {
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              rs::LDS_BYTES);
+    (void)hipGetLastError();
+}
+
+extern "C" int fafk_rs_init(void) {
+    rs_attr(kr_r2c<false>); rs_attr(kr_r2c<true>);
+    rs_attr(kr_c2r<false>); rs_attr(kr_c2r<true>);
+    rs_attr(kr_c2c<false, false, false>); rs_attr(kr_c2c<true, false, false>);
+    rs_attr(kr_c2c<false, true, true>); rs_attr(kr_c2c<true, true, true>);
+    return 0;
+}
+
+/* float2 row access needs 8-byte aligned rows */
+extern "C" int fafk_rs_pair_ok(const void *p, long dist) {
+    return ((uintptr_t)p & 7u) == 0 && (dist & 1) == 0;
+}
+
+static unsigned rs_grid(long count, int grid) {
+    long g = grid > 0 ? grid : 256;
+    return (unsigned)(count < g ? count : g);
+}
+
+#define RS_CHECK(name)                                                    \
+    do {                                                                  \
+        hipError_t e_ = hipGetLastError();                                \
+        if (e_ != hipSuccess) return rs_fail(name, e_);                   \
+    } while (0)
+
+extern "C" int fafk_rs_r2c(int inter_out, const void *tw, const void *post, const float *x,
+                           long in_dist, float *out_re, float *out_im, long out_dist,
+                           long count, float scale, int grid, void *stream) {
+    if (count <= 0) return 0;
+    const dim3 g(rs_grid(count, grid)), b(rs::T);
+    if (inter_out)
+        hipLaunchKernelGGL(kr_r2c<true>, g, b, rs::LDS_BYTES, S(stream), (const float2 *)tw,
+                           (const float2 *)post, x, in_dist, out_re, out_im, out_dist, count,
+                           scale);
+    else
+        hipLaunchKernelGGL(kr_r2c<false>, g, b, rs::LDS_BYTES, S(stream), (const float2 *)tw,
+                           (const float2 *)post, x, in_dist, out_re, out_im, out_dist, count,
+                           scale);
+    RS_CHECK("kr_r2c");
+    return 0;
+}
+
+extern "C" int fafk_rs_c2r(int inter_in, const void *tw, const void *pre, const float *in_re,
+                           const float *in_im, long in_dist, float *y, long out_dist,
+                           long count, float scale, int grid, void *stream) {
+    if (count <= 0) return 0;
+    const dim3 g(rs_grid(count, grid)), b(rs::T);
+    if (inter_in)
+        hipLaunchKernelGGL(kr_c2r<true>, g, b, rs::LDS_BYTES, S(stream), (const float2 *)tw,
+                           (const float2 *)pre, in_re, in_im, in_dist, y, out_dist, count,
+                           scale);
+    else
+        hipLaunchKernelGGL(kr_c2r<false>, g, b, rs::LDS_BYTES, S(stream), (const float2 *)tw,
+                           (const float2 *)pre, in_re, in_im, in_dist, y, out_dist, count,
+                           scale);
+    RS_CHECK("kr_c2r");
+    return 0;
+}
+
+extern "C" int fafk_rs_c2c(int inv, int inter, const void *tw, const float *in_re,
+                           const float *in_im, long in_dist, float *out_re, float *out_im,
+                           long out_dist, long count, float scale, int grid, void *stream) {
+    if (count <= 0) return 0;
+    const dim3 g(rs_grid(count, grid)), b(rs::T);
+    const float2 *t = (const float2 *)tw;
+#define RS_C2C(I, L)                                                                   \
+    hipLaunchKernelGGL((kr_c2c<I, L, L>), g, b, rs::LDS_BYTES, S(stream), t, in_re, in_im, \
+                       in_dist, out_re, out_im, out_dist, count, scale)
+    if (inv) {
+        if (inter) RS_C2C(true, true); else RS_C2C(true, false);
+    } else {
+        if (inter) RS_C2C(false, true); else RS_C2C(false, false);
+    }
+#undef RS_C2C
+    RS_CHECK("kr_c2c");
+    return 0;
+}

@@ -28,6 +28,13 @@ static int kcerr(int rc) {
     return rc;
 }
 
+static int krerr(int rc) {
+    if (rc) fafi_err("%s", fafk_rs_last_error());
+    return rc;
+}
+
+static int rs_grid(void) { return (int)fafi_cus(); }
+
 static long chunk_signals(size_t n_complex, long count) {
     long c = (long)(CHUNK_BYTES / (n_complex * 2 * sizeof(float)));
     if (c < 1) c = 1;
@@ -38,9 +45,12 @@ size_t fafi_run_ws_bytes(const fafh_plan *p, long count) {
     if (count <= 0) return 0;
     switch (p->kind) {
     case PK_C2C_4S:
+        if (p->rs) return 0;
         return (size_t)chunk_signals(p->n, count) * p->n * 2 * sizeof(float);
     case PK_R2C_4S:
     case PK_C2R_4S:
+        /* with the single pass the workspace is needed only for rows that are
+         * not 8-byte aligned, which fall back to four-step: keep its size */
         if (p->ct4)
             return (size_t)chunk_signals(p->n / 2, count) * (p->n / 2) * 2 * sizeof(float);
         return (size_t)chunk_signals(p->n / 2, count) * (p->n / 2) * 4 * sizeof(float);
@@ -79,6 +89,9 @@ int fafi_run_c2c(fafh_plan *p, const float *ir, const float *ii, long idist, flo
     }
     case PK_C2C_4S: {
         size_t n = p->n;
+        if (p->rs)   /* one pass, no workspace */
+            return krerr(fafk_rs_c2c(p->inverse, inter, p->twn, ir, ii, idist, orr, oi, odist,
+                                     count, p->scale, rs_grid(), stream));
         long cs = chunk_signals(n, count);
         float *ws = fafi_workspace(p, (size_t)cs * n * 2 * sizeof(float));
         if (!ws) return -1;
@@ -128,6 +141,9 @@ int fafi_run_r2c(fafh_plan *p, const float *x, long idist, float *orr, float *oi
                              p->scale, stream));
     if (p->kind != PK_R2C_4S) { fafi_err("plan is not an R2C transform"); return -1; }
     size_t nc = p->n / 2;
+    if (p->rs && fafk_rs_pair_ok(x, idist))   /* one pass, no workspace */
+        return krerr(fafk_rs_r2c(inter, p->twn, p->post, x, idist, orr, oi, odist, count,
+                                 p->scale, rs_grid(), stream));
     long cs = chunk_signals(nc, count);
     if (p->ct4) {
         /* column pass, then the row pass with the Kiss post-pass fused */
@@ -176,6 +192,9 @@ int fafi_run_c2r(fafh_plan *p, const float *ir, const float *ii, long idist, flo
                              p->scale, stream));
     if (p->kind != PK_C2R_4S) { fafi_err("plan is not a C2R transform"); return -1; }
     size_t nc = p->n / 2;
+    if (p->rs && fafk_rs_pair_ok(y, odist))   /* one pass, no workspace */
+        return krerr(fafk_rs_c2r(inter, p->twn, p->pre, ir, ii, idist, y, odist, count,
+                                 p->scale, rs_grid(), stream));
     long cs = chunk_signals(nc, count);
     if (p->ct4) {
         /* Kiss pre-pass fused into the column pass, real pairs out of the row pass */

@@ -523,6 +523,7 @@ static int plan_4step(fafh_plan *p, size_t n, int inv) {
     p->cols = c;
     p->rows = r;
     p->ct4 = fafi_ct_enabled() && fafk_ct4_supported(n1, n2);
+    p->rs = fafi_ct_enabled() && n == 32768;
     return 0;
 }
 

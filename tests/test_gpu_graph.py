@@ -98,8 +98,8 @@ def test_graph_c1_forward_inverse_replays():
 
 
 @pytest.mark.parametrize("n", [65536, 16384])
-def test_graph_rfft_irfft_four_step_replays(n):
-    """Plans with several kernels and a workspace (four-step at 65536)."""
+def test_graph_rfft_irfft_replays(n):
+    """rfft then irfft in one graph (65536: the single-pass kernels)."""
     say(f"case: graph rfft+irfft n={n}")
     lib = G.L()
     b = 6
@@ -141,6 +141,37 @@ def test_graph_rfft_irfft_four_step_replays(n):
     finally:
         lib.faf_graph_destroy(g)
         G.destroy(ti)
+        G.destroy(t)
+
+
+def test_graph_c2c_four_step_replays():
+    """C2C 65536 runs as a column pass and a row pass per workspace chunk:
+    a graph of several kernels sharing the plan's workspace."""
+    say("case: graph c2c 65536 four-step")
+    lib = G.L()
+    n, b = 65536, 5
+    rng = np.random.default_rng(n + 9)
+    t = G.create("fft", n)
+    xr, xi, yr, yi, er, ei = (G.empty((b, n)) for _ in range(6))
+    bx = fa.buffer_split(xr.data_ptr(), xi.data_ptr(), n)
+    by = fa.buffer_split(yr.data_ptr(), yi.data_ptr(), n)
+    be = fa.buffer_split(er.data_ptr(), ei.data_ptr(), n)
+    g = None
+    try:
+        fa.check(lib.faf_execute_batch(t, C.byref(by), C.byref(bx), b, 0, 0), "eager")
+        g = capture(lambda: [lib.faf_execute_batch(t, C.byref(by), C.byref(bx), b, 0, 0)])
+        assert lib.faf_graph_nodes(g) >= 2
+        for rep in range(3):
+            x = cplx(rng, b, n)
+            put(xr, x.real)
+            put(xi, x.imag)
+            fa.check(lib.faf_execute_batch(t, C.byref(be), C.byref(bx), b, 0, 0), "eager")
+            fa.check(lib.faf_graph_launch(g), "graph launch")
+            y = G.host(yr) + 1j * G.host(yi)
+            assert np.array_equal(y, G.host(er) + 1j * G.host(ei)), f"replay {rep}"
+            assert rel_l2(y, O.fft64(x.astype(np.complex128))) < tol(n)
+    finally:
+        lib.faf_graph_destroy(g)
         G.destroy(t)
 
 
@@ -190,7 +221,7 @@ def test_graph_refusals_leave_state_clean():
     n = 1024
     nb = n // 2 + 1
     t = G.create("rfft", n)
-    big = G.create("rfft", 65536)          # never executed: no workspace yet
+    big = G.create("fft", 65536)           # four-step, never executed: no workspace yet
     try:
         # host buffers cannot be captured
         xh = np.random.default_rng(3).standard_normal(n).astype(np.float32)
@@ -209,9 +240,9 @@ def test_graph_refusals_leave_state_clean():
         assert lib.faf_graph_begin() == -1                 # no nesting
         assert not lib.faf_graph_end()                     # nothing captured
         # a workspace cannot grow inside a capture
-        xd, red, imd = G.empty((2, 65536)), G.empty((2, 32769)), G.empty((2, 32769))
-        bi2 = fa.buffer_real(xd.data_ptr(), 65536)
-        bo2 = fa.buffer_hermitian(red.data_ptr(), imd.data_ptr(), 32769)
+        xr, xi, yr, yi = (G.empty((2, 65536)) for _ in range(4))
+        bi2 = fa.buffer_split(xr.data_ptr(), xi.data_ptr(), 65536)
+        bo2 = fa.buffer_split(yr.data_ptr(), yi.data_ptr(), 65536)
         fa.check(lib.faf_graph_begin(), "begin")
         assert lib.faf_execute_batch(big, C.byref(bo2), C.byref(bi2), 2, 0, 0) == -1
         assert "workspace" in err()

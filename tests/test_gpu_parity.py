@@ -106,7 +106,7 @@ def test_c2c_norms(norm):
         G.destroy(ti)
 
 
-@pytest.mark.parametrize("n", [64, 4096, 65536])
+@pytest.mark.parametrize("n", [64, 4096, 32768, 65536])
 def test_c2c_interleaved(n):
     say(f"case: c2c interleaved n={n}")
     rng = np.random.default_rng(5)
@@ -313,6 +313,58 @@ def test_c2_full_batch_round_trip(n):
         G.destroy(ti)
         del x, re, im, y
         torch.cuda.empty_cache()
+
+
+def test_single_pass_32768_many_signals():
+    """C2C 32768 runs in one pass, one workgroup per CU looping over the
+    signals: 300 signals make the first 44 workgroups take two."""
+    say("case: c2c 32768 x 300")
+    n, b = 32768, 300
+    rng = np.random.default_rng(n)
+    x = cplx(rng, b, n)
+    t = G.create("fft", n)
+    ti = G.create("fft", n, dir="inverse")
+    try:
+        y = G.c2c(t, x)
+        sel = [0, 1, 43, 44, 255, 256, 299]
+        assert rel_l2(y[sel], truth_fft(x[sel])) < tol(n)
+        z = G.c2c(ti, y)
+        assert rel_l2(z, x) < 2 * tol(n)
+    finally:
+        G.destroy(t)
+        G.destroy(ti)
+    t = G.create("fft", n, layout="interleaved")
+    try:
+        yi = G.c2c(t, x[:2], inter=True)
+        assert rel_l2(yi, truth_fft(x[:2])) < tol(n)
+    finally:
+        G.destroy(t)
+
+
+def test_rfft_65536_unaligned_rows_fall_back():
+    """Rows at an odd float distance are not 8-byte aligned, so the rfft
+    takes the four-step path instead of the single pass; both agree with
+    FP64."""
+    say("case: rfft 65536 odd row distance")
+    lib = G.L()
+    n, cnt = 65536, 3
+    nb = n // 2 + 1
+    rng = np.random.default_rng(65537)
+    xs = rng.standard_normal((cnt, n + 1)).astype(np.float32)
+    t = G.create("rfft", n)
+    try:
+        dx = G.dev(xs)
+        re, im = G.empty((cnt, nb)), G.empty((cnt, nb))
+        torch.cuda.synchronize()
+        bi = fa.buffer_real(dx.data_ptr(), n)
+        bo = fa.buffer_hermitian(re.data_ptr(), im.data_ptr(), nb)
+        fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), cnt, n + 1, 0), "rfft")
+        X = G.host(re) + 1j * G.host(im)
+        truth = np.stack([O.fft64(r[:n].astype(np.float64))[0][:nb] for r in xs])
+        assert rel_l2(X, truth) < tol(n)
+        assert rel_l2(G.r2c(t, np.ascontiguousarray(xs[:, :n])), truth) < tol(n)
+    finally:
+        G.destroy(t)
 
 
 def test_batch_member_equals_alone():
