@@ -551,6 +551,17 @@ struct GFrameIn {            /* windowed frame packed as z[i] = (w x)[2i] + i (w
     }
 };
 
+/* the same when every frame of the launch lies inside its record (record
+ * length >= window): no bounds test, and the pairs load as 8-byte words */
+struct GFrameFull {
+    const float *x;
+    const float *w;
+    static constexpr bool is_lds = false;
+    FAF_DEV float2 ld(int i) const {
+        return make_float2(x[2 * i] * w[2 * i], x[2 * i + 1] * w[2 * i + 1]);
+    }
+};
+
 #ifndef FAF_ENGINE_NO_PLAIN_KERNELS
 template <int MT>
 __global__ __launch_bounds__(MT) void k_stft(const fafd_fft *Pd, int spb, const float2 *post, const float *win,

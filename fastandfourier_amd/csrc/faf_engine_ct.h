@@ -282,7 +282,8 @@ void kc_pipe(fafd_fft Pf, fafd_fft Pi, const float2 *post, const float2 *pre, fa
     bw.run(lds, lds, dst, lt, active);
 }
 
-template <int NC>
+/* FULL: every frame lies inside its record (record length >= window) */
+template <bool FULL, int NC>
 __global__ __launch_bounds__((Ct<false, NC>::T * ct_spb<Ct<false, NC>::T>()))
 void kc_stft(fafd_fft P, const float2 *post, const float *win, const float *x, long rec_len,
              long rec_dist, long hop, long frames, float *out_re, float *out_im,
@@ -294,7 +295,6 @@ void kc_stft(fafd_fft P, const float2 *post, const float *win, const float *x, l
     const long sig = active ? g : 0;
     Lds lds{faf_smem + sid * P.plane};
     const long rec = sig / frames, f = sig - rec * frames;
-    GFrameIn src{x + rec * rec_dist + f * hop, win, rec_len - f * hop};
     typename Ct<false, NC>::Chain ch;
     ch.load_tw(lt, active, (const float2 *)P.tw);
     constexpr int NP = (NC / 2 + T) / T;
@@ -304,7 +304,13 @@ void kc_stft(fafd_fft P, const float2 *post, const float *win, const float *x, l
         const int k = lt + i * T;
         if (active && k > 0 && k <= NC / 2) pw[i] = post[k];
     }
-    ch.run(src, lds, lds, lt, active);
+    if constexpr (FULL) {
+        GFrameFull src{x + rec * rec_dist + f * hop, win};
+        ch.run(src, lds, lds, lt, active);
+    } else {
+        GFrameIn src{x + rec * rec_dist + f * hop, win, rec_len - f * hop};
+        ch.run(src, lds, lds, lt, active);
+    }
     __syncthreads();
     if (!active) return;
     float *ore = out_re + rec * rec_out_dist + f * frame_dist;

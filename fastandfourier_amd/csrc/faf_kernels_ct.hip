@@ -43,7 +43,7 @@ template <int N> static void ct_attrs_n() {
     ct_attr(kc_c2c<false, true, N>); ct_attr(kc_c2c<true, true, N>);
     ct_attr(kc_r2c<false, N>); ct_attr(kc_r2c<true, N>);
     ct_attr(kc_c2r<false, N>); ct_attr(kc_c2r<true, N>);
-    ct_attr(kc_pipe<N>); ct_attr(kc_stft<N>);
+    ct_attr(kc_pipe<N>); ct_attr(kc_stft<false, N>); ct_attr(kc_stft<true, N>);
 }
 
 template <int N> static void ct_attrs_cwt() {
@@ -162,7 +162,14 @@ extern "C" int fafk_ct_stft(const fafd_fft *P, const void *post, const float *wi
                             float *out_re, float *out_im, long frame_dist, long rec_out_dist,
                             long count, void *stream) {
     const float2 *po = (const float2 *)post;
-#define STFT(N) hipLaunchKernelGGL((kc_stft<N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, po, win, x, rec_len, rec_dist, hop, frames, out_re, out_im, frame_dist, rec_out_dist, count)
+    /* with a record at least one window long every frame is whole (the
+     * 32768-sample window keeps the checked loader: the other spills) */
+    const bool full = rec_len >= 2L * P->n && P->n <= 8192;
+#define STFT(N)                                                                          \
+    do {                                                                                 \
+        if (full) hipLaunchKernelGGL((kc_stft<true, N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, po, win, x, rec_len, rec_dist, hop, frames, out_re, out_im, frame_dist, rec_out_dist, count); \
+        else hipLaunchKernelGGL((kc_stft<false, N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, po, win, x, rec_len, rec_dist, hop, frames, out_re, out_im, frame_dist, rec_out_dist, count); \
+    } while (0)
     CT_SWITCH(P->n, STFT)
 #undef STFT
     hipError_t e = hipGetLastError();
