@@ -49,6 +49,7 @@ static void rt_init(void) {
     g_rt_rc = fafk_init();
     if (!g_rt_rc) g_rt_rc = fafk_ct_init();
     if (!g_rt_rc) g_rt_rc = fafk_rs_init();
+    if (!g_rt_rc) g_rt_rc = fafk_rg_init();
     if (g_rt_rc) {
         snprintf(g_rt_msg, sizeof g_rt_msg, "%s", fafk_last_error());
         return;

@@ -994,45 +994,8 @@ __global__ __launch_bounds__(MT) void k_blue(const fafd_fft *Pfd,
 /* atomic queue; signals live at offsets[] in concatenated split planes */
 /* ------------------------------------------------------------------ */
 
-#ifndef FAF_ENGINE_NO_PLAIN_KERNELS
-template <int MT>
-__global__ __launch_bounds__(MT) void k_ragged(const fafd_rplan *plans, const fafd_unit *units, int nunits,
-                         const int32_t *order, const long *offsets, const float *in_re,
-                         const float *in_im, float *out_re, float *out_im,
-                         unsigned *counter) {
-    __shared__ int s_unit;
-    const int tid = (int)threadIdx.x;
-    for (;;) {
-        if (tid == 0) s_unit = (int)atomicAdd(counter, 1u);
-        __syncthreads();
-        const int u = s_unit;
-        __syncthreads();
-        if (u >= nunits) break;
-        const fafd_unit un = units[u];
-        const fafd_rplan *pl = plans + un.plan;
-        const int T = pl->threads;
-        const int sid = tid / T, lt = tid - sid * T;
-        const int active = sid < un.count;
-        const int s = active ? order[un.first + sid] : order[un.first];
-        const long off = offsets[s];
-        GSplitIn src{in_re + off, in_im + off};
-        GSplitOut dst{out_re + off, out_im + off, 1.0f};
-        if (pl->kind == FAFD_KIND_STOCKHAM) {
-            Lds lds{faf_smem + sid * pl->fwd.plane};
-            fft_run<false>(pl->fwd, src, lds, dst, lt, active);
-        } else if (pl->kind == FAFD_KIND_RADER) {
-            const int per = pl->fwd.plane + lpad(pl->rader.p) + 1;
-            Lds xa{faf_smem + sid * per};
-            Lds aa{faf_smem + sid * per + lpad(pl->rader.p) + 1};
-            rader_body(pl->fwd, pl->inv, pl->rader, src, dst, xa, aa, lt, T, active);
-        } else {
-            Lds aa{faf_smem + sid * pl->fwd.plane};
-            blue_body(pl->fwd, pl->inv, pl->blue, src, dst, aa, lt, active);
-        }
-        __syncthreads();
-    }
-}
-#endif
+/* the kernel itself, with compile-time passes for power-of-two lengths, is
+ * kg_ragged in faf_kernels_rg.hip */
 
 
 /* y = window * x on interleaved complex rows (single-frame STFT) */

@@ -92,6 +92,8 @@ int fafk_rader(const fafd_fft *Pf, const fafd_fft *Pi, const fafd_rader *R, int 
 int fafk_blue(const fafd_fft *Pf, const fafd_fft *Pi, const fafd_blue *Bq, int spb,
               const float *in_re, const float *in_im, long in_dist, float *out_re,
               float *out_im, long out_dist, long count, float scale, void *stream);
+int fafk_rg_init(void);   /* faf_kernels_rg.hip */
+const char *fafk_rg_last_error(void);
 int fafk_ragged(const void *plans, const void *units, int nunits, const int32_t *order,
                 const long *offsets, const float *in_re, const float *in_im, float *out_re,
                 float *out_im, unsigned *counter, int grid, int block, size_t lds,

@@ -49,7 +49,7 @@ template <int M> static void init_attrs() {
     lds_attr(k_c2r<false, M>); lds_attr(k_c2r<true, M>);
     lds_attr(k_pipe<M>); lds_attr(k_stft<M>);
     lds_attr(k_cwt<false, M>); lds_attr(k_cwt<true, M>);
-    lds_attr(k_rader<M>); lds_attr(k_blue<M>); lds_attr(k_ragged<M>); lds_attr(k_icwt<M>);
+    lds_attr(k_rader<M>); lds_attr(k_blue<M>); lds_attr(k_icwt<M>);
     lds_attr(k_serve<M>);
 }
 
@@ -529,20 +529,6 @@ extern "C" int fafk_blue(const fafd_fft *Pf, const fafd_fft *Pi, const fafd_blue
     BY_MT(b.x, hipLaunchKernelGGL(k_blue<MT>, g, b, lds, S(stream), Pfd, Pid, *Bq, spb, in_re,
                                   in_im, in_dist, out_re, out_im, out_dist, count, scale));
     LAUNCH_CHECK("k_blue");
-    return 0;
-}
-
-extern "C" int fafk_ragged(const void *plans, const void *units, int nunits,
-                           const int32_t *order, const long *offsets, const float *in_re,
-                           const float *in_im, float *out_re, float *out_im,
-                           unsigned *counter, int grid, int block, size_t lds,
-                           void *stream) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, S(stream), counter, 1, 0u);
-    LAUNCH_CHECK("k_fill_u32");
-    BY_MT(block, hipLaunchKernelGGL(k_ragged<MT>, dim3(grid), dim3(block), lds, S(stream),
-                                    (const fafd_rplan *)plans, (const fafd_unit *)units, nunits,
-                                    order, offsets, in_re, in_im, out_re, out_im, counter));
-    LAUNCH_CHECK("k_ragged");
     return 0;
 }
 

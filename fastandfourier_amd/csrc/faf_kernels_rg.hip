@@ -1,0 +1,119 @@
+/*
+ * faf_kernels_rg.hip -- the ragged persistent launch: C2C signals of uneven
+ * lengths in one launch (host side: faf_ragged.c).
+ *
+ * Workgroups pull units (up to spb same-length signals) from an atomic
+ * counter until the queue is empty.  A unit of power-of-two length
+ * 256 ... 8192 runs the compile-time passes of that length (T = N/16
+ * threads per signal, faf_engine_ct.h); other lengths run the plan's
+ * runtime passes, Rader or Bluestein.  The length switch is uniform across
+ * the workgroup, so the barriers inside every path stay uniform too.
+ */
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include "faf_engine_ct.h"
+#include "faf_kern.h"
+
+static thread_local char g_rg_err[160];
+
+static int rg_fail(const char *what, hipError_t e) {
+    snprintf(g_rg_err, sizeof g_rg_err, "%s: %s", what, hipGetErrorString(e));
+    return -1;
+}
+
+static inline hipStream_t S(void *s) { return (hipStream_t)s; }
+
+__global__ void kg_fill_u32(unsigned *p, int n, unsigned v) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = v;
+}
+
+template <int N>
+FAF_DEV void rg_ct(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst, int sid,
+                   int lt, int active) {
+    Lds lds{faf_smem + sid * pl->fwd.plane};
+    typename Ct<false, N>::Chain ch;
+    ch.load_tw(lt, active, (const float2 *)pl->fwd.tw);
+    ch.run(src, lds, dst, lt, active);
+}
+
+template <int MT>
+__global__ __launch_bounds__(MT) void kg_ragged(const fafd_rplan *plans, const fafd_unit *units,
+                                                int nunits, const int32_t *order,
+                                                const long *offsets, const float *in_re,
+                                                const float *in_im, float *out_re,
+                                                float *out_im, unsigned *counter) {
+    __shared__ int s_unit;
+    const int tid = (int)threadIdx.x;
+    for (;;) {
+        if (tid == 0) s_unit = (int)atomicAdd(counter, 1u);
+        __syncthreads();
+        const int u = s_unit;
+        __syncthreads();
+        if (u >= nunits) break;
+        const fafd_unit un = units[u];
+        const fafd_rplan *pl = plans + un.plan;
+        const int T = pl->threads;
+        const int sid = tid / T, lt = tid - sid * T;
+        const int active = sid < un.count;
+        const int s = active ? order[un.first + sid] : order[un.first];
+        const long off = offsets[s];
+        GSplitIn src{in_re + off, in_im + off};
+        GSplitOut dst{out_re + off, out_im + off, 1.0f};
+        if (pl->kind == FAFD_KIND_CT) {
+            switch (pl->len) {
+            case 256: rg_ct<256>(pl, src, dst, sid, lt, active); break;
+            case 512: rg_ct<512>(pl, src, dst, sid, lt, active); break;
+            case 1024: rg_ct<1024>(pl, src, dst, sid, lt, active); break;
+            case 2048: rg_ct<2048>(pl, src, dst, sid, lt, active); break;
+            case 4096: rg_ct<4096>(pl, src, dst, sid, lt, active); break;
+            case 8192: rg_ct<8192>(pl, src, dst, sid, lt, active); break;
+            default: {                  /* not set by the host: runtime passes */
+                Lds lds{faf_smem + sid * pl->fwd.plane};
+                fft_run<false>(pl->fwd, src, lds, dst, lt, active);
+            }
+            }
+        } else if (pl->kind == FAFD_KIND_STOCKHAM) {
+            Lds lds{faf_smem + sid * pl->fwd.plane};
+            fft_run<false>(pl->fwd, src, lds, dst, lt, active);
+        } else if (pl->kind == FAFD_KIND_RADER) {
+            const int per = pl->fwd.plane + lpad(pl->rader.p) + 1;
+            Lds xa{faf_smem + sid * per};
+            Lds aa{faf_smem + sid * per + lpad(pl->rader.p) + 1};
+            rader_body(pl->fwd, pl->inv, pl->rader, src, dst, xa, aa, lt, T, active);
+        } else {
+            Lds aa{faf_smem + sid * pl->fwd.plane};
+            blue_body(pl->fwd, pl->inv, pl->blue, src, dst, aa, lt, active);
+        }
+        __syncthreads();
+    }
+}
+
+extern "C" int fafk_rg_init(void) {
+    (void)hipFuncSetAttribute((const void *)kg_ragged<512>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              FAFD_MAX_LDS_BYTES - 64);
+    (void)hipGetLastError();
+    return 0;
+}
+
+extern "C" int fafk_ragged(const void *plans, const void *units, int nunits,
+                           const int32_t *order, const long *offsets, const float *in_re,
+                           const float *in_im, float *out_re, float *out_im,
+                           unsigned *counter, int grid, int block, size_t lds,
+                           void *stream) {
+    hipLaunchKernelGGL(kg_fill_u32, dim3(1), dim3(64), 0, S(stream), counter, 1, 0u);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return rg_fail("kg_fill_u32", e);
+    if (block > 512) {               /* the host plans every unit for 512 threads */
+        snprintf(g_rg_err, sizeof g_rg_err, "ragged launch of %d threads per workgroup", block);
+        return -1;
+    }
+    hipLaunchKernelGGL(kg_ragged<512>, dim3(grid), dim3(block), lds, S(stream),
+                       (const fafd_rplan *)plans, (const fafd_unit *)units, nunits, order,
+                       offsets, in_re, in_im, out_re, out_im, counter);
+    e = hipGetLastError();
+    return e == hipSuccess ? 0 : rg_fail("kg_ragged", e);
+}
+
+extern "C" const char *fafk_rg_last_error(void) { return g_rg_err; }

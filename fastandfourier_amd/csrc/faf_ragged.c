@@ -83,8 +83,12 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
         d->len = (int32_t)uniq[k];
         size_t per;
         if (p->kind == PK_C2C) {
-            d->kind = FAFD_KIND_STOCKHAM;
+            /* power-of-two lengths with compile-time passes: N/16 threads */
+            const size_t n = uniq[k];
+            const int ct = fafi_ct_enabled() && n >= 256 && n <= 8192 && !(n & (n - 1));
+            d->kind = ct ? FAFD_KIND_CT : FAFD_KIND_STOCKHAM;
             d->fwd = p->a;
+            if (ct) d->fwd.threads = (int32_t)(n / 16);
             per = (size_t)p->a.plane;
         } else if (p->kind == PK_RADER) {
             d->kind = FAFD_KIND_RADER;
@@ -147,7 +151,8 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
     for (int k = 0; k < nu; ++k) {
         double lg = 1.0;
         for (size_t m = uniq[k]; m > 1; m >>= 1) lg += 1.0;
-        double per_sig = (double)uniq[k] * lg * (tab[k].kind == FAFD_KIND_STOCKHAM ? 1.0 : 2.5);
+        const int stockham = tab[k].kind == FAFD_KIND_STOCKHAM || tab[k].kind == FAFD_KIND_CT;
+        double per_sig = (double)uniq[k] * lg * (stockham ? 1.0 : 2.5);
         for (int s = 0; s < nper[k]; s += tab[k].spb) {
             int c = nper[k] - s < tab[k].spb ? nper[k] - s : tab[k].spb;
             uc[nuu].u.plan = k;
@@ -199,6 +204,6 @@ int faf_ragged_execute(faf_ragged *r, float *out_re, float *out_im, const float 
     int rc = fafk_ragged(r->d_plans, r->d_units, r->nunits, r->d_order, r->d_off, in_re, in_im,
                          out_re, out_im, r->d_counter, r->grid, RAGGED_BLOCK, r->lds, stream);
     if (!rc && fafi_sync_mode()) rc = fafk_sync(stream);
-    if (rc) fafi_err("ragged: %s", fafk_last_error());
+    if (rc) fafi_err("ragged: %s", fafk_rg_last_error());
     return rc;
 }
