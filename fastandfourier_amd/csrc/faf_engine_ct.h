@@ -106,6 +106,21 @@ template <> struct Pow2Seq<4096> { using type = Radices<16, 16, 16>; };
 template <> struct Pow2Seq<8192> { using type = Radices<16, 16, 16, 2>; };
 template <> struct Pow2Seq<16384> { using type = Radices<16, 16, 16, 4>; };
 
+/* 5-smooth lengths with compile-time passes in the ragged launch: the
+ * runtime planner's radix order (powers of two first, then 3s, then 5s) and
+ * its thread count, so one unit's layout is the same either way */
+template <int N> struct MixSeq { static constexpr int T = 0; };
+template <> struct MixSeq<4800> { using type = Radices<16, 4, 3, 5, 5>; static constexpr int T = 320; };
+template <> struct MixSeq<3000> { using type = Radices<8, 3, 5, 5, 5>; static constexpr int T = 200; };
+template <> struct MixSeq<1000> { using type = Radices<8, 5, 5, 5>; static constexpr int T = 67; };
+template <> struct MixSeq<360> { using type = Radices<8, 3, 3, 5>; static constexpr int T = 24; };
+template <> struct MixSeq<60> { using type = Radices<4, 3, 5>; static constexpr int T = 4; };
+
+template <bool INV, int N, int T, class Seq> struct MixFFT;
+template <bool INV, int N, int T, int... Rs> struct MixFFT<INV, N, T, Radices<Rs...>> {
+    using Chain = CtChain<INV, N, T, 1, Rs...>;
+};
+
 template <bool INV, int N, class Seq> struct CtFFT;
 template <bool INV, int N, int... Rs> struct CtFFT<INV, N, Radices<Rs...>> {
     static constexpr int T = N / 16;

@@ -4,9 +4,10 @@
  *
  * Workgroups pull units (up to spb same-length signals) from an atomic
  * counter until the queue is empty.  A unit of power-of-two length
- * 256 ... 8192 runs the compile-time passes of that length (T = N/16
- * threads per signal, faf_engine_ct.h); other lengths run the plan's
- * runtime passes, Rader or Bluestein.  The length switch is uniform across
+ * 256 ... 8192 (T = N/16 threads per signal) or of one of the 5-smooth
+ * lengths of MixSeq runs the compile-time passes of that length
+ * (faf_engine_ct.h); other lengths run the plan's runtime passes, Rader or
+ * Bluestein.  The length switch is uniform across
  * the workgroup, so the barriers inside every path stay uniform too.
  */
 #include <hip/hip_runtime.h>
@@ -33,6 +34,15 @@ FAF_DEV void rg_ct(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &d
                    int lt, int active) {
     Lds lds{faf_smem + sid * pl->fwd.plane};
     typename Ct<false, N>::Chain ch;
+    ch.load_tw(lt, active, (const float2 *)pl->fwd.tw);
+    ch.run(src, lds, dst, lt, active);
+}
+
+template <int N>
+FAF_DEV void rg_mix(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst, int sid,
+                    int lt, int active) {
+    Lds lds{faf_smem + sid * pl->fwd.plane};
+    typename MixFFT<false, N, MixSeq<N>::T, typename MixSeq<N>::type>::Chain ch;
     ch.load_tw(lt, active, (const float2 *)pl->fwd.tw);
     ch.run(src, lds, dst, lt, active);
 }
@@ -68,6 +78,11 @@ __global__ __launch_bounds__(MT) void kg_ragged(const fafd_rplan *plans, const f
             case 2048: rg_ct<2048>(pl, src, dst, sid, lt, active); break;
             case 4096: rg_ct<4096>(pl, src, dst, sid, lt, active); break;
             case 8192: rg_ct<8192>(pl, src, dst, sid, lt, active); break;
+            case 60: rg_mix<60>(pl, src, dst, sid, lt, active); break;
+            case 360: rg_mix<360>(pl, src, dst, sid, lt, active); break;
+            case 1000: rg_mix<1000>(pl, src, dst, sid, lt, active); break;
+            case 3000: rg_mix<3000>(pl, src, dst, sid, lt, active); break;
+            case 4800: rg_mix<4800>(pl, src, dst, sid, lt, active); break;
             default: {                  /* not set by the host: runtime passes */
                 Lds lds{faf_smem + sid * pl->fwd.plane};
                 fft_run<false>(pl->fwd, src, lds, dst, lt, active);
@@ -117,3 +132,17 @@ extern "C" int fafk_ragged(const void *plans, const void *units, int nunits,
 }
 
 extern "C" const char *fafk_rg_last_error(void) { return g_rg_err; }
+
+/* threads per signal of the compile-time passes of length n in the ragged
+ * launch, 0 if n has none */
+extern "C" int fafk_rg_ct_threads(int n) {
+    switch (n) {
+    case 256: case 512: case 1024: case 2048: case 4096: case 8192: return n / 16;
+    case 60: return MixSeq<60>::T;
+    case 360: return MixSeq<360>::T;
+    case 1000: return MixSeq<1000>::T;
+    case 3000: return MixSeq<3000>::T;
+    case 4800: return MixSeq<4800>::T;
+    default: return 0;
+    }
+}

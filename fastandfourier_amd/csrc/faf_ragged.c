@@ -83,12 +83,15 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
         d->len = (int32_t)uniq[k];
         size_t per;
         if (p->kind == PK_C2C) {
-            /* power-of-two lengths with compile-time passes: N/16 threads */
+            /* lengths with compile-time passes in the ragged kernel; the
+             * 5-smooth ones share the runtime plan's thread count */
             const size_t n = uniq[k];
-            const int ct = fafi_ct_enabled() && n >= 256 && n <= 8192 && !(n & (n - 1));
+            const int ct_t = fafi_ct_enabled() && n <= 8192 ? fafk_rg_ct_threads((int)n) : 0;
+            const int pow2 = !(n & (n - 1));
+            const int ct = ct_t > 0 && (pow2 || ct_t == p->a.threads);
             d->kind = ct ? FAFD_KIND_CT : FAFD_KIND_STOCKHAM;
             d->fwd = p->a;
-            if (ct) d->fwd.threads = (int32_t)(n / 16);
+            if (ct) d->fwd.threads = ct_t;
             per = (size_t)p->a.plane;
         } else if (p->kind == PK_RADER) {
             d->kind = FAFD_KIND_RADER;
