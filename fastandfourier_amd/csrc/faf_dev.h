@@ -50,7 +50,8 @@ typedef struct fafd_ops {
 
 /* plan kinds of the persistent ragged launch */
 enum { FAFD_KIND_STOCKHAM = 0, FAFD_KIND_RADER = 1, FAFD_KIND_BLUESTEIN = 2,
-       FAFD_KIND_CT = 3 /* Stockham through the compile-time passes of its power-of-two length */ };
+       FAFD_KIND_CT = 3,        /* Stockham through compile-time passes of its length */
+       FAFD_KIND_RADER_CT = 4   /* Rader with compile-time inner passes */ };
 
 typedef struct fafd_rader {
     int32_t p;           /* prime length */

@@ -29,6 +29,10 @@ template <bool INV, int N, int T, int NS, int... Rs> struct CtChain {
     FAF_DEV void load_tw(int, int, const float2 *) {}
 };
 
+/* run_lazy() is run() with each pass's twiddle loads issued beside that
+ * pass's own data reads instead of all at the start: fewer registers held
+ * across the passes, for kernels whose occupancy is register-bound. */
+
 template <bool INV, int N, int T, int NS, int R, int... Rest>
 struct CtChain<INV, N, T, NS, R, Rest...> {
     static constexpr int NB = N / R, B = (NB + T - 1) / T, K = TwBits<R>::K;
@@ -91,6 +95,56 @@ struct CtChain<INV, N, T, NS, R, Rest...> {
             next.run(lds, lds, dst, lt, active);
         }
     }
+
+    template <class S, class D>
+    FAF_DEV void stage_lazy(const S &src, const D &dst, int lt, int active, bool sync,
+                            const float2 *__restrict__ tw) {
+        float2 v[B][R];
+        float2 tl[NS > 1 ? B : 1][NS > 1 ? K : 1];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = lt + b * T;
+            if (active && j < NB) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) v[b][t] = src.ld(j + t * NB);
+                if constexpr (NS > 1) {
+                    const int step = (j % NS) * (N / (NS * R));
+#pragma unroll
+                    for (int i = 0; i < K; ++i) tl[b][i] = tw[step << i];
+                }
+            }
+        }
+        if (sync) __syncthreads();
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int j = lt + b * T;
+            if (active && j < NB) {
+                const int k = j % NS;
+                if constexpr (NS > 1) {
+                    float2 w[R];
+                    tw_expand<R>(tl[b], w);
+#pragma unroll
+                    for (int t = 1; t < R; ++t) v[b][t] = cmul(v[b][t], w[t]);
+                }
+                dft<R, INV>(v[b]);
+                const int base = (j - k) * R + k;
+#pragma unroll
+                for (int t = 0; t < R; ++t) dst.st(base + t * NS, v[b][t]);
+            }
+        }
+    }
+
+    template <class S, class D>
+    FAF_DEV void run_lazy(const S &src, const Lds &lds, const D &dst, int lt, int active,
+                          const float2 *tw) {
+        if constexpr (sizeof...(Rest) == 0) {
+            stage_lazy(src, dst, lt, active, S::is_lds || (!S::is_lds && !D::is_lds), tw);
+        } else {
+            stage_lazy(src, lds, lt, active, S::is_lds, tw);
+            __syncthreads();
+            next.run_lazy(lds, lds, dst, lt, active, tw);
+        }
+    }
 };
 
 /* radix sequence of a power-of-two length, as a type */
@@ -115,6 +169,16 @@ template <> struct MixSeq<3000> { using type = Radices<8, 3, 5, 5, 5>; static co
 template <> struct MixSeq<1000> { using type = Radices<8, 5, 5, 5>; static constexpr int T = 67; };
 template <> struct MixSeq<360> { using type = Radices<8, 3, 3, 5>; static constexpr int T = 24; };
 template <> struct MixSeq<60> { using type = Radices<4, 3, 5>; static constexpr int T = 4; };
+
+/* Rader primes of the ragged launch with compile-time inner passes over
+ * p - 1 points: the runtime planner's radix order for p - 1 (7s allowed) and
+ * threads per signal */
+template <int P> struct RaderSeq { static constexpr int T = 0; };
+template <> struct RaderSeq<11> { using type = Radices<2, 5>; static constexpr int T = 1; };
+template <> struct RaderSeq<17> { using type = Radices<16>; static constexpr int T = 1; };
+template <> struct RaderSeq<31> { using type = Radices<2, 3, 5>; static constexpr int T = 2; };
+template <> struct RaderSeq<97> { using type = Radices<16, 2, 3>; static constexpr int T = 7; };
+template <> struct RaderSeq<1009> { using type = Radices<16, 3, 3, 7>; static constexpr int T = 72; };
 
 template <bool INV, int N, int T, class Seq> struct MixFFT;
 template <bool INV, int N, int T, int... Rs> struct MixFFT<INV, N, T, Radices<Rs...>> {

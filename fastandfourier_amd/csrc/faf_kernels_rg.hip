@@ -6,8 +6,9 @@
  * counter until the queue is empty.  A unit of power-of-two length
  * 256 ... 8192 (T = N/16 threads per signal) or of one of the 5-smooth
  * lengths of MixSeq runs the compile-time passes of that length
- * (faf_engine_ct.h); other lengths run the plan's runtime passes, Rader or
- * Bluestein.  The length switch is uniform across
+ * (faf_engine_ct.h), and a Rader prime of RaderSeq runs Rader with
+ * compile-time inner passes; other lengths run the plan's runtime passes,
+ * Rader or Bluestein.  The length switch is uniform across
  * the workgroup, so the barriers inside every path stay uniform too.
  */
 #include <hip/hip_runtime.h>
@@ -29,13 +30,14 @@ __global__ void kg_fill_u32(unsigned *p, int n, unsigned v) {
     if (i < n) p[i] = v;
 }
 
+/* Passes issue their twiddle loads lazily (CtChain::run_lazy): the kernel's
+ * occupancy is set by its register count, not by latency inside one pass. */
 template <int N>
 FAF_DEV void rg_ct(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst, int sid,
                    int lt, int active) {
     Lds lds{faf_smem + sid * pl->fwd.plane};
     typename Ct<false, N>::Chain ch;
-    ch.load_tw(lt, active, (const float2 *)pl->fwd.tw);
-    ch.run(src, lds, dst, lt, active);
+    ch.run_lazy(src, lds, dst, lt, active, (const float2 *)pl->fwd.tw);
 }
 
 template <int N>
@@ -43,11 +45,42 @@ FAF_DEV void rg_mix(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &
                     int lt, int active) {
     Lds lds{faf_smem + sid * pl->fwd.plane};
     typename MixFFT<false, N, MixSeq<N>::T, typename MixSeq<N>::type>::Chain ch;
-    ch.load_tw(lt, active, (const float2 *)pl->fwd.tw);
-    ch.run(src, lds, dst, lt, active);
+    ch.run_lazy(src, lds, dst, lt, active, (const float2 *)pl->fwd.tw);
 }
 
-template <int MT>
+/* Rader for prime P with compile-time passes over the P - 1 inner points;
+ * the same steps as rader_body (faf_engine.h) */
+template <int P>
+FAF_DEV void rg_rader(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst,
+                      int sid, int lt, int active) {
+    constexpr int T = RaderSeq<P>::T;
+    using Seq = typename RaderSeq<P>::type;
+    const fafd_rader &R = pl->rader;
+    const int per = pl->fwd.plane + lpad(P) + 1;
+    Lds xa{faf_smem + sid * per};
+    Lds aa{faf_smem + sid * per + lpad(P) + 1};
+    if (active)
+        for (int i = lt; i < P; i += T) xa.st(i, src.ld(i));
+    __syncthreads();
+    const float2 x0 = active ? xa.ld(0) : make_float2(0.0f, 0.0f);
+    GatherLds g{xa, R.rin};
+    typename MixFFT<false, P - 1, T, Seq>::Chain fw;
+    fw.run_lazy(g, aa, aa, lt, active, (const float2 *)pl->fwd.tw);
+    __syncthreads();
+    if (active && lt == 0) xa.st(0, cadd(x0, aa.ld(0)));   /* DC = x0 + sum x[1..] */
+    MulLds mb{aa, (const float2 *)R.b};
+    ScatterLds sc{xa, R.rout, x0};
+    typename MixFFT<true, P - 1, T, Seq>::Chain bw;
+    bw.run_lazy(mb, aa, sc, lt, active, (const float2 *)pl->inv.tw);
+    __syncthreads();
+    if (active)
+        for (int i = lt; i < P; i += T) dst.st(i, xa.ld(i));
+}
+
+/* GEN: also the runtime Stockham, Rader and Bluestein paths.  Without them
+ * the kernel needs far fewer registers (more workgroups per CU); the host
+ * picks it when every length of the batch has compile-time passes. */
+template <int MT, bool GEN>
 __global__ __launch_bounds__(MT) void kg_ragged(const fafd_rplan *plans, const fafd_unit *units,
                                                 int nunits, const int32_t *order,
                                                 const long *offsets, const float *in_re,
@@ -83,29 +116,40 @@ __global__ __launch_bounds__(MT) void kg_ragged(const fafd_rplan *plans, const f
             case 1000: rg_mix<1000>(pl, src, dst, sid, lt, active); break;
             case 3000: rg_mix<3000>(pl, src, dst, sid, lt, active); break;
             case 4800: rg_mix<4800>(pl, src, dst, sid, lt, active); break;
-            default: {                  /* not set by the host: runtime passes */
+            default: break;             /* the host sets this kind for the lengths above only */
+            }
+        } else if (pl->kind == FAFD_KIND_RADER_CT) {
+            switch (pl->len) {
+            case 11: rg_rader<11>(pl, src, dst, sid, lt, active); break;
+            case 17: rg_rader<17>(pl, src, dst, sid, lt, active); break;
+            case 31: rg_rader<31>(pl, src, dst, sid, lt, active); break;
+            case 97: rg_rader<97>(pl, src, dst, sid, lt, active); break;
+            case 1009: rg_rader<1009>(pl, src, dst, sid, lt, active); break;
+            default: break;
+            }
+        } else if constexpr (GEN) {
+            if (pl->kind == FAFD_KIND_STOCKHAM) {
                 Lds lds{faf_smem + sid * pl->fwd.plane};
                 fft_run<false>(pl->fwd, src, lds, dst, lt, active);
+            } else if (pl->kind == FAFD_KIND_RADER) {
+                const int per = pl->fwd.plane + lpad(pl->rader.p) + 1;
+                Lds xa{faf_smem + sid * per};
+                Lds aa{faf_smem + sid * per + lpad(pl->rader.p) + 1};
+                rader_body(pl->fwd, pl->inv, pl->rader, src, dst, xa, aa, lt, T, active);
+            } else {
+                Lds aa{faf_smem + sid * pl->fwd.plane};
+                blue_body(pl->fwd, pl->inv, pl->blue, src, dst, aa, lt, active);
             }
-            }
-        } else if (pl->kind == FAFD_KIND_STOCKHAM) {
-            Lds lds{faf_smem + sid * pl->fwd.plane};
-            fft_run<false>(pl->fwd, src, lds, dst, lt, active);
-        } else if (pl->kind == FAFD_KIND_RADER) {
-            const int per = pl->fwd.plane + lpad(pl->rader.p) + 1;
-            Lds xa{faf_smem + sid * per};
-            Lds aa{faf_smem + sid * per + lpad(pl->rader.p) + 1};
-            rader_body(pl->fwd, pl->inv, pl->rader, src, dst, xa, aa, lt, T, active);
-        } else {
-            Lds aa{faf_smem + sid * pl->fwd.plane};
-            blue_body(pl->fwd, pl->inv, pl->blue, src, dst, aa, lt, active);
         }
         __syncthreads();
     }
 }
 
 extern "C" int fafk_rg_init(void) {
-    (void)hipFuncSetAttribute((const void *)kg_ragged<512>,
+    (void)hipFuncSetAttribute((const void *)kg_ragged<512, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              FAFD_MAX_LDS_BYTES - 64);
+    (void)hipFuncSetAttribute((const void *)kg_ragged<512, false>,
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               FAFD_MAX_LDS_BYTES - 64);
     (void)hipGetLastError();
@@ -115,7 +159,7 @@ extern "C" int fafk_rg_init(void) {
 extern "C" int fafk_ragged(const void *plans, const void *units, int nunits,
                            const int32_t *order, const long *offsets, const float *in_re,
                            const float *in_im, float *out_re, float *out_im,
-                           unsigned *counter, int grid, int block, size_t lds,
+                           unsigned *counter, int grid, int block, size_t lds, int generic,
                            void *stream) {
     hipLaunchKernelGGL(kg_fill_u32, dim3(1), dim3(64), 0, S(stream), counter, 1, 0u);
     hipError_t e = hipGetLastError();
@@ -124,14 +168,32 @@ extern "C" int fafk_ragged(const void *plans, const void *units, int nunits,
         snprintf(g_rg_err, sizeof g_rg_err, "ragged launch of %d threads per workgroup", block);
         return -1;
     }
-    hipLaunchKernelGGL(kg_ragged<512>, dim3(grid), dim3(block), lds, S(stream),
-                       (const fafd_rplan *)plans, (const fafd_unit *)units, nunits, order,
-                       offsets, in_re, in_im, out_re, out_im, counter);
+    if (generic)
+        hipLaunchKernelGGL((kg_ragged<512, true>), dim3(grid), dim3(block), lds, S(stream),
+                           (const fafd_rplan *)plans, (const fafd_unit *)units, nunits, order,
+                           offsets, in_re, in_im, out_re, out_im, counter);
+    else
+        hipLaunchKernelGGL((kg_ragged<512, false>), dim3(grid), dim3(block), lds, S(stream),
+                           (const fafd_rplan *)plans, (const fafd_unit *)units, nunits, order,
+                           offsets, in_re, in_im, out_re, out_im, counter);
     e = hipGetLastError();
     return e == hipSuccess ? 0 : rg_fail("kg_ragged", e);
 }
 
 extern "C" const char *fafk_rg_last_error(void) { return g_rg_err; }
+
+/* threads per signal of the Rader path for prime p with compile-time inner
+ * passes, 0 if p has none */
+extern "C" int fafk_rg_rader_threads(int p) {
+    switch (p) {
+    case 11: return RaderSeq<11>::T;
+    case 17: return RaderSeq<17>::T;
+    case 31: return RaderSeq<31>::T;
+    case 97: return RaderSeq<97>::T;
+    case 1009: return RaderSeq<1009>::T;
+    default: return 0;
+    }
+}
 
 /* threads per signal of the compile-time passes of length n in the ragged
  * launch, 0 if n has none */

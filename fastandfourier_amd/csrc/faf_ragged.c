@@ -15,6 +15,8 @@
 
 #define RAGGED_BLOCK 512
 #define RAGGED_LDS_CAP (96 * 1024)
+/* with compile-time paths only, two workgroups share a CU's 160 KB */
+#define RAGGED_LDS_CAP_CT (79 * 1024)
 
 struct faf_ragged {
     size_t count, total;
@@ -22,6 +24,7 @@ struct faf_ragged {
     fafh_plan **hp;           /* host plans own the device tables */
     void *d_plans, *d_units, *d_order, *d_off, *d_counter;
     int nunits;
+    int generic;              /* some length needs the runtime paths */
     int grid;
     size_t lds;
 };
@@ -73,6 +76,7 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
         if (i == 0 || uniq[i] != uniq[i - 1]) uniq[nu++] = uniq[i];
     r->hp = calloc((size_t)nu, sizeof(fafh_plan *));
     fafd_rplan *tab = calloc((size_t)nu, sizeof(fafd_rplan));
+    size_t *pers = calloc((size_t)nu, sizeof(size_t));
     int ok = 1;
     size_t lds_max = 0;
     for (int k = 0; k < nu && ok; ++k) {
@@ -94,8 +98,10 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
             if (ct) d->fwd.threads = ct_t;
             per = (size_t)p->a.plane;
         } else if (p->kind == PK_RADER) {
-            d->kind = FAFD_KIND_RADER;
+            const int rt = fafi_ct_enabled() ? fafk_rg_rader_threads((int)uniq[k]) : 0;
+            d->kind = rt > 0 ? FAFD_KIND_RADER_CT : FAFD_KIND_RADER;
             d->fwd = p->a; d->inv = p->b; d->rader = p->rader;
+            if (rt > 0) d->fwd.threads = rt;
             per = (size_t)p->a.plane + uniq[k] + (uniq[k] >> 4) + 1;
         } else if (p->kind == PK_BLUE) {
             d->kind = FAFD_KIND_BLUESTEIN;
@@ -112,8 +118,16 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
             ok = 0;
             break;
         }
+        if (d->kind != FAFD_KIND_CT && d->kind != FAFD_KIND_RADER_CT) r->generic = 1;
+        pers[k] = per;
+    }
+    /* signals per unit: as many as the threads and the LDS cap allow */
+    const size_t cap = r->generic ? RAGGED_LDS_CAP : RAGGED_LDS_CAP_CT;
+    for (int k = 0; k < nu && ok; ++k) {
+        fafd_rplan *d = &tab[k];
+        const size_t per = pers[k];
         int spb = RAGGED_BLOCK / d->threads;
-        while (spb > 1 && (size_t)spb * per * 8 > RAGGED_LDS_CAP) --spb;
+        while (spb > 1 && (size_t)spb * per * 8 > cap) --spb;
         if ((size_t)spb * per * 8 > FAFD_MAX_LDS_BYTES) {
             fafi_err("ragged: length %zu does not fit LDS", uniq[k]);
             ok = 0;
@@ -122,6 +136,7 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
         d->spb = spb;
         if ((size_t)spb * per * 8 > lds_max) lds_max = (size_t)spb * per * 8;
     }
+    free(pers);
     if (!ok) { free(uniq); free(tab); faf_ragged_destroy(r); return NULL; }
 
     /* order signals by plan, offsets as exclusive prefix sums */
@@ -205,7 +220,8 @@ int faf_ragged_execute(faf_ragged *r, float *out_re, float *out_im, const float 
         return -1;
     }
     int rc = fafk_ragged(r->d_plans, r->d_units, r->nunits, r->d_order, r->d_off, in_re, in_im,
-                         out_re, out_im, r->d_counter, r->grid, RAGGED_BLOCK, r->lds, stream);
+                         out_re, out_im, r->d_counter, r->grid, RAGGED_BLOCK, r->lds, r->generic,
+                         stream);
     if (!rc && fafi_sync_mode()) rc = fafk_sync(stream);
     if (rc) fafi_err("ragged: %s", fafk_rg_last_error());
     return rc;
