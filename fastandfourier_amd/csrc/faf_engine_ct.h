@@ -316,7 +316,19 @@ void kc_c2r(fafd_fft P, const float2 *pre, const float *in_re, const float *in_i
     ch.run(lds, lds, dst, lt, active);
 }
 
-template <int NC>
+/* the op list that is exactly one MUL by the bound spectrum (the notch and
+ * filter pipelines): apply_ops without the per-bin interpretation */
+FAF_DEV float2 op_mul(const fafd_ops &ops, int k, float2 v) {
+    const long m = ops.h_bins;
+    if (k < m) {
+        if (k == 0 || k == m - 1) v = make_float2(v.x * ops.h_re[k], 0.0f);
+        else v = cmul(v, make_float2(ops.h_re[k], ops.h_im[k]));
+    }
+    return v;
+}
+
+/* MUL1: ops is one MUL (op_mul); otherwise the general apply_ops */
+template <int NC, bool MUL1>
 __global__ __launch_bounds__((Ct<false, NC>::T * ct_spb<Ct<false, NC>::T>()))
 void kc_pipe(fafd_fft Pf, fafd_fft Pi, const float2 *post, const float2 *pre, fafd_ops ops,
              const float *x, long in_dist, float *y, long out_dist, long count, float scale) {
@@ -340,15 +352,16 @@ void kc_pipe(fafd_fft Pf, fafd_fft Pi, const float2 *post, const float2 *pre, fa
             if (k > NC / 2) break;
             const int m = NC - k;
             float2 xk, xm;
+            auto op = [&](int b, float2 v) { return MUL1 ? op_mul(ops, b, v) : apply_ops(ops, b, v); };
             if (k == 0) {
                 float2 z = lds.ld(0);
-                xk = apply_ops(ops, 0, make_float2(z.x + z.y, 0.0f));
-                xm = apply_ops(ops, NC, make_float2(z.x - z.y, 0.0f));
+                xk = op(0, make_float2(z.x + z.y, 0.0f));
+                xm = op(NC, make_float2(z.x - z.y, 0.0f));
                 lds.st(0, make_float2(xk.x + xm.x, xk.x - xm.x));
             } else {
                 kiss_post(lds.ld(k), lds.ld(m), post[k], xk, xm);
-                xk = apply_ops(ops, k, xk);
-                xm = (m != k) ? apply_ops(ops, m, xm) : xk;
+                xk = op(k, xk);
+                xm = (m != k) ? op(m, xm) : xk;
                 float2 fk, fm;
                 kiss_pre(xk, xm, pre[k], fk, fm);
                 lds.st(k, fk);

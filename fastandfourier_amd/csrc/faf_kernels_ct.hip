@@ -43,7 +43,8 @@ template <int N> static void ct_attrs_n() {
     ct_attr(kc_c2c<false, true, N>); ct_attr(kc_c2c<true, true, N>);
     ct_attr(kc_r2c<false, N>); ct_attr(kc_r2c<true, N>);
     ct_attr(kc_c2r<false, N>); ct_attr(kc_c2r<true, N>);
-    ct_attr(kc_pipe<N>); ct_attr(kc_stft<false, N>); ct_attr(kc_stft<true, N>);
+    ct_attr(kc_pipe<N, false>); ct_attr(kc_pipe<N, true>);
+    ct_attr(kc_stft<false, N>); ct_attr(kc_stft<true, N>);
 }
 
 template <int N> static void ct_attrs_cwt() {
@@ -150,7 +151,12 @@ extern "C" int fafk_ct_pipe(const fafd_fft *Pf, const fafd_fft *Pi, const void *
                             const void *pre, const fafd_ops *ops, const float *x, long in_dist,
                             float *y, long out_dist, long count, float scale, void *stream) {
     const float2 *po = (const float2 *)post, *pr = (const float2 *)pre;
-#define PIPE(N) hipLaunchKernelGGL((kc_pipe<N>), GRID(N), BLOCK(N), LDS(N, Pf->plane), S(stream), *Pf, *Pi, po, pr, *ops, x, in_dist, y, out_dist, count, scale)
+    const bool mul1 = ops->nops == 1 && ops->code[0] == FAFD_OP_MUL;
+#define PIPE(N)                                                                          \
+    do {                                                                                 \
+        if (mul1) hipLaunchKernelGGL((kc_pipe<N, true>), GRID(N), BLOCK(N), LDS(N, Pf->plane), S(stream), *Pf, *Pi, po, pr, *ops, x, in_dist, y, out_dist, count, scale); \
+        else hipLaunchKernelGGL((kc_pipe<N, false>), GRID(N), BLOCK(N), LDS(N, Pf->plane), S(stream), *Pf, *Pi, po, pr, *ops, x, in_dist, y, out_dist, count, scale); \
+    } while (0)
     CT_SWITCH(Pf->n, PIPE)
 #undef PIPE
     hipError_t e = hipGetLastError();
