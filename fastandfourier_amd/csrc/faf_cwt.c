@@ -561,9 +561,14 @@ int fafi_run_cwt(fafh_plan *p, const float *x, long idist, float *out, long odis
     if (p->sub_r2c) return run_cwt_long(p, x, idist, out, odist, count, complex_out, stream);
     int rows = p->rows_total;
     /* enough workgroups to fill the device: split the rows of a segment
-     * across workgroups when there are few segments */
+     * across workgroups when there are few segments; and at most 11 rows per
+     * workgroup, so the last round of workgroups is short (each group
+     * recomputes the segment's rfft, a small cost next to 11 inverse rows).
+     * Measured on 1024 segments of 4096 with 65 rows: 1 group 0.70 ms,
+     * 3 groups 0.60, 6 groups (11 rows) 0.56, 11 groups 0.61. */
     long want = 2 * (long)fafi_cus();
     int groups = (int)((want + count - 1) / count);
+    if (groups < (rows + 10) / 11) groups = (rows + 10) / 11;
     if (groups < 1) groups = 1;
     if (groups > rows) groups = rows;
     int rpw = (rows + groups - 1) / groups;
