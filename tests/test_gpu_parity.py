@@ -501,6 +501,66 @@ def test_c3_ragged_persistent_launch():
         lib.faf_ragged_destroy(r)
 
 
+def test_ragged_runtime_paths():
+    """Lengths without compile-time passes (and a Bluestein length) put the
+    launch on the kernel that carries the runtime Stockham, Rader and
+    Bluestein paths; every signal still matches FP64."""
+    say("case: ragged runtime paths")
+    lib = G.L()
+    rng = np.random.default_rng(12)
+    pool = [4050, 128, 13, 19, 2000, 23, 97, 8192, 24, 1009, 60, 4096]
+    lens = rng.choice(pool, size=400)
+    arr = (C.c_size_t * len(lens))(*[int(v) for v in lens])
+    r = lib.faf_ragged_create(arr, len(lens), fa.FLAG_BLUESTEIN)
+    assert r, lib.faf_get_error()
+    try:
+        tot = lib.faf_ragged_total(r)
+        x = (rng.uniform(-1, 1, tot) + 1j * rng.uniform(-1, 1, tot)).astype(np.complex64)
+        dxr, dxi = G.dev(x.real.copy()), G.dev(x.imag.copy())
+        yr, yi = G.empty(tot), G.empty(tot)
+        torch.cuda.synchronize()
+        fa.check(lib.faf_ragged_execute(r, yr.data_ptr(), yi.data_ptr(), dxr.data_ptr(),
+                                        dxi.data_ptr()), "ragged")
+        y = G.host(yr) + 1j * G.host(yi)
+        off = np.concatenate([[0], np.cumsum(lens)])
+        seen = set()
+        for i in range(len(lens)):
+            n = int(lens[i])
+            if n in seen and i % 5:
+                continue
+            seen.add(n)
+            s = slice(off[i], off[i + 1])
+            assert rel_l2(y[s], O.fft64(x[s].astype(np.complex128))[0]) < tol(n), (i, n)
+        assert seen == set(pool)
+    finally:
+        lib.faf_ragged_destroy(r)
+
+
+def test_framed_stft_short_and_exact_records():
+    """A record shorter than the window is one zero-padded frame (the loader
+    with bounds tests); a record exactly one window long is one whole frame."""
+    say("case: stft short and exact records")
+    lib = G.L()
+    win, hop = 1024, 256
+    t = G.create("stft", win, layout="real", win_length=win, hop_length=hop)
+    try:
+        rng = np.random.default_rng(21)
+        nb = win // 2 + 1
+        for length in (700, 1024):
+            x = rng.standard_normal((2, length)).astype(np.float32)
+            assert lib.faf_stft_frames(t, length) == 1
+            dx = G.dev(x)
+            re, im = G.empty((2, 1, nb)), G.empty((2, 1, nb))
+            torch.cuda.synchronize()
+            fa.check(lib.faf_execute_stft(t, re.data_ptr(), im.data_ptr(), dx.data_ptr(),
+                                          length, 2, 0), "stft")
+            S = G.host(re) + 1j * G.host(im)
+            for r in range(2):
+                assert rel_l2(S[r], O.stft64(x[r], win, hop)) < tol(win), length
+    finally:
+        G.destroy(t)
+
+
 # ------------------------------------------------------------------ STFT / CWT (C4)
 
 def test_framed_stft():
