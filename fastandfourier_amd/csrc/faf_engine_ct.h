@@ -322,10 +322,14 @@ void kc_c2r(fafd_fft P, const float2 *pre, const float *in_re, const float *in_i
  * passes leave half the threads idle, but one signal per workgroup halves
  * the LDS per workgroup, and with lazily loaded twiddles (63 VGPRs) eight
  * waves fit per SIMD, twice the staged kernel's.  Measured, rfft 4096:
- * 5.14 -> 5.21 TB/s; irfft 4096: 5.13 -> 5.27 TB/s. */
+ * 5.14 -> 5.21 TB/s; irfft 4096: 5.13 -> 5.27 TB/s; irfft 1024 +3%, rfft
+ * 16384 +1.3%.  The launchers keep the staged kernels where these lose
+ * (rfft 1024 -1%, irfft 16384 -20%). */
 template <int NC> struct WideSeq;
+template <> struct WideSeq<512> { using type = Radices<8, 8, 8>; };
 template <> struct WideSeq<1024> { using type = Radices<8, 16, 8>; };
 template <> struct WideSeq<2048> { using type = Radices<8, 16, 16>; };
+template <> struct WideSeq<8192> { using type = Radices<8, 16, 16, 4>; };
 template <bool INV, int NC> using Wide =
     MixFFT<INV, NC, NC / 8, typename WideSeq<NC>::type>;
 

@@ -69,13 +69,16 @@ template <int N1, int N2> static void ct_attrs_4() {
     ct_attrs_4r<N1, N2, F4_ROWS>();
 }
 
-template <int N> static void ct_attrs_wide() {
+template <int N> static void ct_attrs_r2cw() {
     ct_attr(kc_r2cw<false, N>); ct_attr(kc_r2cw<true, N>);
+}
+template <int N> static void ct_attrs_c2rw() {
     ct_attr(kc_c2rw<false, N>); ct_attr(kc_c2rw<true, N>);
 }
 
 extern "C" int fafk_ct_init(void) {
-    ct_attrs_wide<1024>(); ct_attrs_wide<2048>();
+    ct_attrs_r2cw<1024>(); ct_attrs_r2cw<2048>(); ct_attrs_r2cw<8192>();
+    ct_attrs_c2rw<512>(); ct_attrs_c2rw<1024>(); ct_attrs_c2rw<2048>();
     ct_attrs_4<128, 256>(); ct_attrs_4<256, 256>();
     ct_attrs_cwt<512>(); ct_attrs_cwt<1024>(); ct_attrs_cwt<2048>(); ct_attrs_cwt<4096>();
     ct_attrs_cwt<8192>();
@@ -127,7 +130,9 @@ extern "C" int fafk_ct_r2c(const fafd_fft *P, int inter_out, const void *post, c
                            long in_dist, float *out_re, float *out_im, long out_dist,
                            long count, float scale, void *stream) {
     const float2 *po = (const float2 *)post;
-    if (P->n == 1024 || P->n == 2048) {        /* one signal of NC/8 threads per workgroup */
+    /* one signal of NC/8 threads per workgroup where that measured faster
+     * (rfft 2048, 4096, 16384; the staged kernel wins at 1024) */
+    if (P->n == 1024 || P->n == 2048 || P->n == 8192) {
         const dim3 g((unsigned)count), b((unsigned)(P->n / 8));
         const size_t lds = (size_t)P->plane * sizeof(float2);
 #define R2CW(N)                                                                          \
@@ -135,7 +140,9 @@ extern "C" int fafk_ct_r2c(const fafd_fft *P, int inter_out, const void *post, c
             if (inter_out) hipLaunchKernelGGL((kc_r2cw<true, N>), g, b, lds, S(stream), *P, po, x, in_dist, out_re, out_im, out_dist, count, scale); \
             else hipLaunchKernelGGL((kc_r2cw<false, N>), g, b, lds, S(stream), *P, po, x, in_dist, out_re, out_im, out_dist, count, scale); \
         } while (0)
-        if (P->n == 1024) R2CW(1024); else R2CW(2048);
+        if (P->n == 1024) R2CW(1024);
+        else if (P->n == 2048) R2CW(2048);
+        else R2CW(8192);
 #undef R2CW
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? 0 : ct_fail("kc_r2cw", e);
@@ -155,7 +162,9 @@ extern "C" int fafk_ct_c2r(const fafd_fft *P, int inter_in, const void *pre, con
                            const float *in_im, long in_dist, float *y, long out_dist, long count,
                            float scale, void *stream) {
     const float2 *pr = (const float2 *)pre;
-    if (P->n == 1024 || P->n == 2048) {        /* one signal of NC/8 threads per workgroup */
+    /* one signal of NC/8 threads per workgroup where that measured faster
+     * (irfft 1024, 2048, 4096; the staged kernel wins at 16384) */
+    if (P->n == 512 || P->n == 1024 || P->n == 2048) {
         const dim3 g((unsigned)count), b((unsigned)(P->n / 8));
         const size_t lds = (size_t)P->plane * sizeof(float2);
 #define C2RW(N)                                                                          \
@@ -163,7 +172,9 @@ extern "C" int fafk_ct_c2r(const fafd_fft *P, int inter_in, const void *pre, con
             if (inter_in) hipLaunchKernelGGL((kc_c2rw<true, N>), g, b, lds, S(stream), *P, pr, in_re, in_im, in_dist, y, out_dist, count, scale); \
             else hipLaunchKernelGGL((kc_c2rw<false, N>), g, b, lds, S(stream), *P, pr, in_re, in_im, in_dist, y, out_dist, count, scale); \
         } while (0)
-        if (P->n == 1024) C2RW(1024); else C2RW(2048);
+        if (P->n == 512) C2RW(512);
+        else if (P->n == 1024) C2RW(1024);
+        else C2RW(2048);
 #undef C2RW
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? 0 : ct_fail("kc_c2rw", e);
