@@ -207,7 +207,7 @@ def test_error_paths():
 
 # ------------------------------------------------------------------ R2C / C2R
 
-@pytest.mark.parametrize("n", [4, 8, 30, 64, 1000, 1024, 4096, 16384, 65536])
+@pytest.mark.parametrize("n", [4, 8, 30, 64, 1000, 1024, 2048, 4096, 8192, 16384, 65536])
 def test_rfft_irfft(n):
     say(f"case: rfft n={n}")
     rng = np.random.default_rng(n + 1)
@@ -226,7 +226,7 @@ def test_rfft_irfft(n):
         G.destroy(ti)
 
 
-@pytest.mark.parametrize("n", [1024, 4096, 65536])
+@pytest.mark.parametrize("n", [1024, 2048, 4096, 16384, 65536])
 @pytest.mark.parametrize("norm", ["none", "ortho", "forward"])
 def test_rfft_interleaved_layout_and_norms(n, norm):
     """INTERLEAVED spectra (nb complex64) and every norm, including the
