@@ -8,6 +8,8 @@
  */
 
 #include "faf_oracle.h"
+#include "fastandfourier.h"   /* the reference's ABI (identical header), for orc_time_ref_rfft_batch */
+#include <dlfcn.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdlib.h>
@@ -900,4 +902,76 @@ double orc_time_rfft_batch(size_t n, size_t count, int threads, const float *x,
     double dt = now_s() - t0;
     free(th); free(jb);
     return dt;
+}
+
+/* The same measurement through the reference library itself, compiled from
+ * its own sources into oracle/_ref (see the Makefile): one plan per thread,
+ * created before the clock starts, and one faf_execute per signal, as a
+ * reference program calls it (src/faf_rfft.c:268-335).  backend is the
+ * reference's faf_backend (0 = AUTO: JIT above its threshold, else VM). */
+typedef faf_transform *(*ref_create_fn)(const faf_config *);
+typedef int (*ref_exec_fn)(const faf_transform *, faf_buffer *, const faf_buffer *);
+typedef void (*ref_destroy_fn)(faf_transform *);
+
+typedef struct {
+    size_t n, lo, hi;
+    const float *x;
+    float *re, *im;
+    int backend, failed;
+    ref_create_fn create;
+    ref_exec_fn exec;
+    ref_destroy_fn destroy;
+    pthread_barrier_t *ready;
+} rjob_t;
+
+static void *ref_worker(void *arg) {
+    rjob_t *j = arg;
+    const size_t nb = j->n / 2 + 1;
+    faf_config cfg = faf_config_init(j->n);
+    cfg.backend = (faf_backend)j->backend;
+    faf_transform *t = j->create(&cfg);
+    pthread_barrier_wait(j->ready);
+    if (!t) { j->failed = 1; return NULL; }
+    for (size_t s = j->lo; s < j->hi; s++) {
+        faf_buffer in = faf_buffer_real((float *)(j->x + s * j->n), j->n);
+        faf_buffer out = faf_buffer_hermitian(j->re + s * nb, j->im + s * nb, nb);
+        if (j->exec(t, &out, &in)) { j->failed = 1; break; }
+    }
+    j->destroy(t);
+    return NULL;
+}
+
+double orc_time_ref_rfft_batch(const char *so, size_t n, size_t count, int threads,
+                               int backend, const float *x, float *re, float *im) {
+    void *h = dlopen(so, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return -1.0;
+    ref_create_fn create = (ref_create_fn)dlsym(h, "faf_create_rfft");
+    ref_exec_fn exec = (ref_exec_fn)dlsym(h, "faf_execute");
+    ref_destroy_fn destroy = (ref_destroy_fn)dlsym(h, "faf_destroy_transform");
+    if (!create || !exec || !destroy) return -1.0;
+    if (threads < 1) threads = 1;
+    pthread_barrier_t ready;
+    pthread_barrier_init(&ready, NULL, (unsigned)threads + 1);
+    pthread_t *th = malloc((size_t)threads * sizeof(pthread_t));
+    rjob_t *jb = calloc((size_t)threads, sizeof(rjob_t));
+    for (int t = 0; t < threads; t++) {
+        jb[t].n = n; jb[t].x = x; jb[t].re = re; jb[t].im = im;
+        jb[t].lo = count * (size_t)t / (size_t)threads;
+        jb[t].hi = count * (size_t)(t + 1) / (size_t)threads;
+        jb[t].backend = backend;
+        jb[t].create = create; jb[t].exec = exec; jb[t].destroy = destroy;
+        jb[t].ready = &ready;
+        pthread_create(&th[t], NULL, ref_worker, &jb[t]);
+    }
+    pthread_barrier_wait(&ready);              /* every plan exists: start the clock */
+    double t0 = now_s();
+    int failed = 0;
+    for (int t = 0; t < threads; t++) {
+        pthread_join(th[t], NULL);
+        failed |= jb[t].failed;
+    }
+    double dt = now_s() - t0;
+    pthread_barrier_destroy(&ready);
+    free(th); free(jb);
+    return failed ? -1.0 : dt;
 }

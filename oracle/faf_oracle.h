@@ -84,6 +84,9 @@ int orc_cwt_complex_f64(size_t n, size_t rows, const float *psi, size_t stride,
  * POSIX threads; returns wall seconds of the timed repeat. */
 double orc_time_rfft_batch(size_t n, size_t count, int threads, const float *x,
                            float *re, float *im);
+/* the same through the compiled reference library at path `so` (-1 on failure) */
+double orc_time_ref_rfft_batch(const char *so, size_t n, size_t count, int threads,
+                               int backend, const float *x, float *re, float *im);
 
 #ifdef __cplusplus
 }

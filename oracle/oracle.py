@@ -64,6 +64,8 @@ def lib():
             "orc_cwt_real_f32": (C.c_int, [_sz, _sz, _f, _sz, _f, _f]),
             "orc_cwt_complex_f64": (C.c_int, [_sz, _sz, _f, _sz, _f, _d, _d]),
             "orc_time_rfft_batch": (C.c_double, [_sz, _sz, C.c_int, _f, _f, _f]),
+            "orc_time_ref_rfft_batch": (C.c_double, [C.c_char_p, _sz, _sz, C.c_int, C.c_int,
+                                                     _f, _f, _f]),
         }
         for k, (r, a) in sig.items():
             fn = getattr(h, k)
@@ -233,6 +235,22 @@ def time_rfft_batch(x: np.ndarray, threads: int):
     re = np.empty(cnt * nb, np.float32)
     im = np.empty(cnt * nb, np.float32)
     dt = lib().orc_time_rfft_batch(n, cnt, threads, x.ravel(), re, im)
+    return dt, (re + 1j * im).reshape(cnt, nb)
+
+
+def time_ref_rfft_batch(x: np.ndarray, threads: int, backend: int = 0):
+    """CPU baseline through the compiled reference (oracle/_ref): seconds for
+    the rfft of every row of x, one plan per thread, one faf_execute per
+    signal; backend 0 = the reference's AUTO (JIT above its threshold)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    cnt, n = x.shape
+    nb = n // 2 + 1
+    re = np.empty(cnt * nb, np.float32)
+    im = np.empty(cnt * nb, np.float32)
+    dt = lib().orc_time_ref_rfft_batch(REF_SO.encode(), n, cnt, threads, backend, x.ravel(),
+                                       re, im)
+    if dt < 0:
+        raise RuntimeError("the compiled reference could not run the rfft")
     return dt, (re + 1j * im).reshape(cnt, nb)
 
 
