@@ -232,3 +232,20 @@ def test_cwt_oracle_against_reference_rows(golden):
 def test_oracle_matches_live_reference(n):
     x = cplx(n).astype(np.complex64)
     assert rel_l2(O.fft32(x), O.ref_fft(x)) < tol(n)
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
+def test_cpu_baseline_timer_runs_the_reference():
+    """The CPU-baseline timer drives the compiled reference (AUTO and VM
+    backends) through its C API on several threads: its spectra are the
+    reference's rfft, which matches FP64 and the restatement."""
+    rng = np.random.default_rng(42)
+    x = rng.standard_normal((24, 4096)).astype(np.float32)
+    truth = np.fft.rfft(x.astype(np.float64), axis=1)
+    for backend in (0, 1):
+        dt, y = O.time_ref_rfft_batch(x, 3, backend)
+        assert dt > 0
+        assert rel_l2(y, truth) < tol(4096)
+        assert rel_l2(y, O.ref_rfft(x)) < 1e-6
+    dtp, yp = O.time_rfft_batch(x, 3)
+    assert rel_l2(yp, truth) < tol(4096)
