@@ -227,26 +227,32 @@ def cwt_complex64(x, psi, stride):
     return (zr + 1j * zi).reshape(rows, n)
 
 
-def time_rfft_batch(x: np.ndarray, threads: int):
+def _out_planes(cnt, nb, out):
+    """re/im planes for cnt spectra: `out` (a pair of float32 arrays of at
+    least cnt*nb values, reused across timings) or new ones"""
+    if out is not None and out[0].size >= cnt * nb and out[1].size >= cnt * nb:
+        return out[0][:cnt * nb], out[1][:cnt * nb]
+    return np.empty(cnt * nb, np.float32), np.empty(cnt * nb, np.float32)
+
+
+def time_rfft_batch(x: np.ndarray, threads: int, out=None):
     """CPU baseline: seconds for rfft of every row of x on `threads` threads."""
     x = np.ascontiguousarray(x, dtype=np.float32)
     cnt, n = x.shape
     nb = n // 2 + 1
-    re = np.empty(cnt * nb, np.float32)
-    im = np.empty(cnt * nb, np.float32)
+    re, im = _out_planes(cnt, nb, out)
     dt = lib().orc_time_rfft_batch(n, cnt, threads, x.ravel(), re, im)
     return dt, (re + 1j * im).reshape(cnt, nb)
 
 
-def time_ref_rfft_batch(x: np.ndarray, threads: int, backend: int = 0):
+def time_ref_rfft_batch(x: np.ndarray, threads: int, backend: int = 0, out=None):
     """CPU baseline through the compiled reference (oracle/_ref): seconds for
     the rfft of every row of x, one plan per thread, one faf_execute per
     signal; backend 0 = the reference's AUTO (JIT above its threshold)."""
     x = np.ascontiguousarray(x, dtype=np.float32)
     cnt, n = x.shape
     nb = n // 2 + 1
-    re = np.empty(cnt * nb, np.float32)
-    im = np.empty(cnt * nb, np.float32)
+    re, im = _out_planes(cnt, nb, out)
     dt = lib().orc_time_ref_rfft_batch(REF_SO.encode(), n, cnt, threads, backend, x.ravel(),
                                        re, im)
     if dt < 0:
