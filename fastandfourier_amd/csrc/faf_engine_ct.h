@@ -463,6 +463,47 @@ void kc_pipe(fafd_fft Pf, fafd_fft Pi, const float2 *post, const float2 *pre, fa
     bw.run(lds, lds, dst, lt, active);
 }
 
+/* The single-MUL pipeline with one signal per workgroup of NC/8 threads
+ * (the kc_r2cw layout for both FFTs, twiddles per pass) */
+template <int NC>
+__global__ __launch_bounds__(NC / 8)
+void kc_pipew(fafd_fft Pf, fafd_fft Pi, const float2 *post, const float2 *pre, fafd_ops ops,
+              const float *x, long in_dist, float *y, long out_dist, long count, float scale) {
+    constexpr int T = NC / 8;
+    const int lt = (int)threadIdx.x;
+    const long sig = blockIdx.x;
+    Lds lds{faf_smem};
+    GPairIn src{x + sig * in_dist};
+    typename Wide<false, NC>::Chain fw;
+    fw.run_lazy(src, lds, lds, lt, 1, (const float2 *)Pf.tw);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < (NC / 2 + T) / T; ++i) {
+        const int k = lt + i * T;
+        if (k > NC / 2) break;
+        const int m = NC - k;
+        float2 xk, xm;
+        if (k == 0) {
+            float2 z = lds.ld(0);
+            xk = op_mul(ops, 0, make_float2(z.x + z.y, 0.0f));
+            xm = op_mul(ops, NC, make_float2(z.x - z.y, 0.0f));
+            lds.st(0, make_float2(xk.x + xm.x, xk.x - xm.x));
+        } else {
+            kiss_post(lds.ld(k), lds.ld(m), post[k], xk, xm);
+            xk = op_mul(ops, k, xk);
+            xm = (m != k) ? op_mul(ops, m, xm) : xk;
+            float2 fk, fm;
+            kiss_pre(xk, xm, pre[k], fk, fm);
+            lds.st(k, fk);
+            if (m != k) lds.st(m, fm);
+        }
+    }
+    __syncthreads();
+    GPairOut dst{y + sig * out_dist, scale};
+    typename Wide<true, NC>::Chain bw;
+    bw.run_lazy(lds, lds, dst, lt, 1, (const float2 *)Pi.tw);
+}
+
 /* FULL: every frame lies inside its record (record length >= window) */
 template <bool FULL, int NC>
 __global__ __launch_bounds__((Ct<false, NC>::T * ct_spb<Ct<false, NC>::T>()))

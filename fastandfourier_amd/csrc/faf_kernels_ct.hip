@@ -79,6 +79,7 @@ template <int N> static void ct_attrs_c2rw() {
 extern "C" int fafk_ct_init(void) {
     ct_attrs_r2cw<1024>(); ct_attrs_r2cw<2048>(); ct_attrs_r2cw<8192>();
     ct_attrs_c2rw<512>(); ct_attrs_c2rw<1024>(); ct_attrs_c2rw<2048>();
+    ct_attr(kc_pipew<2048>);
     ct_attrs_4<128, 256>(); ct_attrs_4<256, 256>();
     ct_attrs_cwt<512>(); ct_attrs_cwt<1024>(); ct_attrs_cwt<2048>(); ct_attrs_cwt<4096>();
     ct_attrs_cwt<8192>();
@@ -195,6 +196,13 @@ extern "C" int fafk_ct_pipe(const fafd_fft *Pf, const fafd_fft *Pi, const void *
                             float *y, long out_dist, long count, float scale, void *stream) {
     const float2 *po = (const float2 *)post, *pr = (const float2 *)pre;
     const bool mul1 = ops->nops == 1 && ops->code[0] == FAFD_OP_MUL;
+    if (mul1 && Pf->n == 2048) {               /* one signal of 256 threads per workgroup */
+        hipLaunchKernelGGL((kc_pipew<2048>), dim3((unsigned)count), dim3(256),
+                           (size_t)Pf->plane * sizeof(float2), S(stream), *Pf, *Pi, po, pr, *ops,
+                           x, in_dist, y, out_dist, count, scale);
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? 0 : ct_fail("kc_pipew", e);
+    }
 #define PIPE(N)                                                                          \
     do {                                                                                 \
         if (mul1) hipLaunchKernelGGL((kc_pipe<N, true>), GRID(N), BLOCK(N), LDS(N, Pf->plane), S(stream), *Pf, *Pi, po, pr, *ops, x, in_dist, y, out_dist, count, scale); \
