@@ -589,9 +589,14 @@ int fafi_run_icwt(fafh_plan *p, const float *w, long idist, float *y, long odist
                   void *stream) {
     fafi_cwt_bank *b = p->bank;
     if (b->inv_kind == FAF_CWT_INV_L1) {
-        fafi_err("cwt: the L1 one-integral inverse is not on the device path; use the "
-                 "dual-frame inverse");
-        return -1;
+        /* reference src/faf_cwt.c:878-915: weight ln2/V, C from the LP report
+         * (1 when it is not positive), both rounded to FP32 as there */
+        double c = b->report.admissibility_C;
+        if (c <= 0.0) c = 1.0;
+        int rc = fafk_icwt_l1(w, idist, y, odist, (long)b->n, (int)b->n_scales, b->has_lp,
+                              (float)(LN2 / (double)b->cfg.voices), (float)c, count, stream);
+        if (rc) fafi_err("%s", fafk_last_error());
+        return rc;
     }
     int rc = fafk_icwt(&p->a, &p->b, p->post, p->pre, p->dual_dev, p->psi_stride,
                        p->rows_total, w, idist, y, odist, count, p->scale, stream);

@@ -59,7 +59,7 @@ FAF_DEV constexpr float cq(int j) {
 /* exp(-+ 2 pi i k / 64): forward sign -, inverse + */
 template <bool INV> FAF_DEV constexpr float2 w64(int k) {
     k &= 63;
-    float c, s;                                   /* cos, sin of pi k / 32 */
+    float c = 0.0f, s = 0.0f;                     /* cos, sin of pi k / 32 */
     if (k <= 16) { c = cq(k); s = cq(16 - k); }
     else if (k <= 32) { c = -cq(32 - k); s = cq(k - 16); }
     else if (k <= 48) { c = -cq(k - 32); s = -cq(48 - k); }

@@ -74,6 +74,9 @@ int fafk_cwt(const fafd_fft *Ph, const fafd_fft *Pn, int real_out, const void *p
 int fafk_icwt(const fafd_fft *Pf, const fafd_fft *Pi, const void *post, const void *pre,
               const float *dual, long dual_stride, int rows, const float *w, long in_dist,
               float *y, long out_dist, long count, float scale, void *stream);
+/* L1 one-integral inverse CWT over real rows [lowpass?][n_scales wavelet rows] */
+int fafk_icwt_l1(const float *w, long in_dist, float *y, long out_dist, long n, int n_scales,
+                 int has_lp, float dlog, float c, long count, void *stream);
 int fafk_fs_col(const fafd_fft *P1, int inv, int in_kind, int n2, int cols, const void *twn,
                 const float *in_re, const float *in_im, long in_dist, float *t_re,
                 float *t_im, long t_dist, long count, void *stream);

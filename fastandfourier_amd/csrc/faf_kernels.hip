@@ -424,6 +424,48 @@ extern "C" int fafk_icwt(const fafd_fft *Pf, const fafd_fft *Pi, const void *pos
     return 0;
 }
 
+/* L1 one-integral inverse CWT (reference src/faf_cwt.c:878-915): per
+ * sample, the wavelet rows summed in row order with weight dlog = ln2/V,
+ * divided by the admissibility constant C, plus the lowpass row (row 0 when
+ * the bank has one).  One thread per sample; a row's loads are coalesced
+ * and eight rows are in flight before they are summed in order. */
+__global__ __launch_bounds__(256) void k_icwt_l1(const float *__restrict__ w, long in_dist,
+                                                 float *__restrict__ y, long out_dist, long n,
+                                                 int n_scales, int has_lp, float dlog, float c,
+                                                 long bps) {
+    const long seg = blockIdx.x / bps;
+    const long i = (blockIdx.x - seg * bps) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float *ws = w + seg * in_dist + (has_lp ? n : 0) + i;
+    float acc = 0.0f;
+    int j = 0;
+    for (; j + 8 <= n_scales; j += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = ws[(long)(j + u) * n];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u] * dlog;
+    }
+    for (; j < n_scales; ++j) acc += ws[(long)j * n] * dlog;
+    acc = acc / c;
+    if (has_lp) acc += w[seg * in_dist + i];
+    y[seg * out_dist + i] = acc;
+}
+
+extern "C" int fafk_icwt_l1(const float *w, long in_dist, float *y, long out_dist, long n,
+                            int n_scales, int has_lp, float dlog, float c, long count,
+                            void *stream) {
+    const long bps = (n + 255) / 256;
+    if (n <= 0 || count <= 0 || n_scales < 0 || bps * count > 0x7fffffffL) {
+        snprintf(g_err, sizeof g_err, "k_icwt_l1: unsupported shape n=%ld count=%ld", n, count);
+        return -1;
+    }
+    hipLaunchKernelGGL(k_icwt_l1, dim3((unsigned)(bps * count)), dim3(256), 0, S(stream), w,
+                       in_dist, y, out_dist, n, n_scales, has_lp, dlog, c, bps);
+    LAUNCH_CHECK("k_icwt_l1");
+    return 0;
+}
+
 extern "C" int fafk_fs_col(const fafd_fft *P1, int inv, int in_kind, int n2, int cols,
                            const void *twn, const float *in_re, const float *in_im,
                            long in_dist, float *t_re, float *t_im, long t_dist, long count,

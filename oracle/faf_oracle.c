@@ -833,6 +833,22 @@ int orc_cwt_real_f32(size_t n, size_t rows, const float *psi, size_t stride,
     return 0;
 }
 
+int orc_icwt_l1_f32(size_t n, size_t n_scales, int has_lp, unsigned voices, double c,
+                    const float *w, float *y) {
+    /* reference src/faf_cwt.c:878-915 (the FP32 branch) */
+    if (!voices) return -1;
+    if (c <= 0.0) c = 1.0;
+    const float dlog = (float)(0.693147180559945309417 / (double)voices), cf = (float)c;
+    const float *rows = w + (has_lp ? n : 0);
+    for (size_t i = 0; i < n; i++) y[i] = 0.0f;
+    for (size_t j = 0; j < n_scales; j++)
+        for (size_t i = 0; i < n; i++) y[i] += rows[j * n + i] * dlog;
+    for (size_t i = 0; i < n; i++) y[i] /= cf;
+    if (has_lp)
+        for (size_t i = 0; i < n; i++) y[i] += w[i];
+    return 0;
+}
+
 int orc_cwt_complex_f64(size_t n, size_t rows, const float *psi, size_t stride,
                         const float *x, double *zr, double *zi) {
     size_t nb = n / 2 + 1;

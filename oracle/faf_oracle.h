@@ -80,6 +80,13 @@ int orc_cwt_real_f32(size_t n, size_t rows, const float *psi, size_t stride,
 int orc_cwt_complex_f64(size_t n, size_t rows, const float *psi, size_t stride,
                         const float *x, double *zr, double *zi);
 
+/* L1 one-integral inverse CWT (reference src/faf_cwt.c:878-915), FP32 as
+ * there: y[i] = (sum over wavelet rows j in order of w_j[i] * (float)(ln2/V))
+ * / (float)C, plus the lowpass row 0 when has_lp.  w holds has_lp + n_scales
+ * rows of n samples.  C <= 0 is taken as 1. */
+int orc_icwt_l1_f32(size_t n, size_t n_scales, int has_lp, unsigned voices, double c,
+                    const float *w, float *y);
+
 /* CPU baseline: batched rfft over `count` signals spread on `threads`
  * POSIX threads; returns wall seconds of the timed repeat. */
 double orc_time_rfft_batch(size_t n, size_t count, int threads, const float *x,

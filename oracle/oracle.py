@@ -63,6 +63,7 @@ def lib():
             "orc_stft_f64": (C.c_int, [_sz, _sz, _sz, _f, _d, _d]),
             "orc_cwt_real_f32": (C.c_int, [_sz, _sz, _f, _sz, _f, _f]),
             "orc_cwt_complex_f64": (C.c_int, [_sz, _sz, _f, _sz, _f, _d, _d]),
+            "orc_icwt_l1_f32": (C.c_int, [_sz, _sz, C.c_int, C.c_uint, C.c_double, _f, _f]),
             "orc_time_rfft_batch": (C.c_double, [_sz, _sz, C.c_int, _f, _f, _f]),
             "orc_time_ref_rfft_batch": (C.c_double, [C.c_char_p, _sz, _sz, C.c_int, C.c_int,
                                                      _f, _f, _f]),
@@ -227,6 +228,18 @@ def cwt_complex64(x, psi, stride):
     return (zr + 1j * zi).reshape(rows, n)
 
 
+def icwt_l1_f32(w, n_scales, has_lp, voices, c):
+    """L1 one-integral inverse of real CWT rows w [has_lp + n_scales, n]
+    (reference src/faf_cwt.c:878-915, FP32)."""
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    n = w.shape[-1]
+    assert w.shape[0] == n_scales + (1 if has_lp else 0)
+    y = np.empty(n, np.float32)
+    rc = lib().orc_icwt_l1_f32(n, n_scales, 1 if has_lp else 0, voices, float(c), w.ravel(), y)
+    assert rc == 0
+    return y
+
+
 def _out_planes(cnt, nb, out):
     """re/im planes for cnt spectra: `out` (a pair of float32 arrays of at
     least cnt*nb values, reused across timings) or new ones"""
@@ -289,6 +302,7 @@ def ref():
             "faf_get_recommended_size": (_sz, [C.c_int, _sz]),
             "faf_cwt_config_init": (fa.FafCwtConfig, [_sz]),
             "faf_create_cwt": (PT, [C.POINTER(fa.FafCwtConfig)]),
+            "faf_create_icwt": (PT, [C.POINTER(fa.FafCwtConfig), C.c_int]),
             "faf_cwt_n_rows": (_sz, [PT]), "faf_cwt_n_scales": (_sz, [PT]),
             "faf_cwt_n_bins": (_sz, [PT]),
             "faf_cwt_psi_f32": (C.POINTER(C.c_float), [PT, _sz]),

@@ -153,6 +153,20 @@ def main():
     fx["cwt512_report"] = rep
     fx["cwt512_in"] = x
     fx["cwt512_rows"] = w.reshape(rows, 512)
+    # the reference's L1 one-integral inverse of those rows (the default bank
+    # is L1-normalised), with the bank's voice count
+    cc = h.faf_cwt_config_init(512)
+    cc.fs = 1.0
+    cc.backend = 1
+    ti = h.faf_create_icwt(C.byref(cc), 1)
+    assert ti, h.faf_get_error()
+    y = np.zeros(512, np.float32)
+    bi = fa.buffer_real(w.ctypes.data, 512)
+    bo = fa.buffer_real(y.ctypes.data, 512)
+    assert h.faf_execute(ti, C.byref(bo), C.byref(bi)) == 0, h.faf_get_error()
+    h.faf_destroy_transform(ti)
+    fx["cwt512_l1_out"] = y
+    fx["cwt512_voices"] = np.array([cc.voices])
     out = os.path.join(HERE, "golden.npz")
     np.savez_compressed(out, **fx)
     print(f"wrote {out}: {len(fx)} arrays, {os.path.getsize(out) / 1e6:.2f} MB")

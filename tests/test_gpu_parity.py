@@ -701,6 +701,59 @@ def test_cwt_c4_segments_and_dual_inverse():
         lib.faf_destroy_transform(ti)
 
 
+def test_icwt_l1_matches_reference_and_round_trips(golden):
+    """L1 one-integral inverse on the device: the compiled reference's
+    output for the default n=512 bank, a batch of three segments against the
+    oracle, and the reference's sine round trip (tests/test_cwt.cpp:172-204,
+    <= 0.25) at n=4096 through the device forward CWT."""
+    say("case: icwt L1")
+    lib = G.L()
+    cc = lib.faf_cwt_config_init(512)
+    cc.fs = 1.0
+    ti = lib.faf_create_icwt(C.byref(cc), 1)
+    assert ti, lib.faf_get_error()
+    try:
+        rows = golden["cwt512_rows"]
+        nr = rows.shape[0]
+        assert lib.faf_cwt_n_rows(ti) == nr
+        rng = np.random.default_rng(21)
+        ws = np.stack([rows, 0.5 * rows, rng.standard_normal(rows.shape).astype(np.float32)])
+        dw = G.dev(ws)
+        y = G.empty((3, 512))
+        torch.cuda.synchronize()
+        bi = fa.buffer_real(dw.data_ptr(), nr * 512)
+        bo = fa.buffer_real(y.data_ptr(), 512)
+        fa.check(lib.faf_execute_batch(ti, C.byref(bo), C.byref(bi), 3, 0, 0), "icwt l1 batch")
+        yh = G.host(y)
+        assert rel_l2(yh[0], golden["cwt512_l1_out"]) < 1e-6
+        v, c = int(golden["cwt512_voices"][0]), golden["cwt512_report"][11]
+        for s in range(3):
+            assert rel_l2(yh[s], O.icwt_l1_f32(ws[s], nr - 1, True, v, c)) < 1e-6
+    finally:
+        lib.faf_destroy_transform(ti)
+    cc = lib.faf_cwt_config_init(4096)
+    t = lib.faf_create_cwt(C.byref(cc))
+    ti = lib.faf_create_icwt(C.byref(cc), 1)
+    assert t and ti, lib.faf_get_error()
+    try:
+        ns, nr = lib.faf_cwt_n_scales(t), lib.faf_cwt_n_rows(t)
+        freqs = np.zeros(ns)
+        lib.faf_cwt_freqs(t, freqs.ctypes.data_as(C.POINTER(C.c_double)), ns)
+        x = np.sin(2 * np.pi * freqs[ns // 2] * np.arange(4096)).astype(np.float32)
+        dx = G.dev(x)
+        w = G.empty((nr, 4096))
+        y = G.empty(4096)
+        torch.cuda.synchronize()
+        fa.check(lib.faf_execute(t, C.byref(fa.buffer_real(w.data_ptr(), nr * 4096)),
+                                 C.byref(fa.buffer_real(dx.data_ptr(), 4096))), "cwt")
+        fa.check(lib.faf_execute(ti, C.byref(fa.buffer_real(y.data_ptr(), 4096)),
+                                 C.byref(fa.buffer_real(w.data_ptr(), nr * 4096))), "icwt l1")
+        assert rel_l2(G.host(y), x) <= 0.25
+    finally:
+        lib.faf_destroy_transform(t)
+        lib.faf_destroy_transform(ti)
+
+
 @pytest.mark.parametrize("n", [256, 1024, 4096, 16384])
 def test_generic_and_specialised_kernels_agree(n):
     """Power-of-two lengths run on compile-time specialised kernels by

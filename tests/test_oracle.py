@@ -227,6 +227,39 @@ def test_cwt_oracle_against_reference_rows(golden):
     assert np.abs(Z[:, 512 // 2 + 1:]).max() < 1e-9 * np.abs(Z).max()
 
 
+def test_icwt_l1_oracle_against_reference(golden):
+    """The L1 one-integral inverse restatement reproduces the compiled
+    reference's FP32 output on the default n=512 bank's forward rows."""
+    w = golden["cwt512_rows"]
+    C_ = golden["cwt512_report"][11]
+    v = int(golden["cwt512_voices"][0])
+    y = O.icwt_l1_f32(w, w.shape[0] - 1, True, v, C_)
+    ref = golden["cwt512_l1_out"]
+    assert rel_l2(y, ref) < 1e-6
+    # each term matters: the lowpass row, the voice weight and C
+    assert rel_l2(O.icwt_l1_f32(w[1:], w.shape[0] - 1, False, v, C_), ref) > 1e-2
+    assert rel_l2(O.icwt_l1_f32(w, w.shape[0] - 1, True, v + 1, C_), ref) > 1e-2
+    assert rel_l2(O.icwt_l1_f32(w, w.shape[0] - 1, True, v, 2 * C_), ref) > 1e-2
+
+
+def test_icwt_l1_oracle_sine_round_trip(golden):
+    """Reference tests/test_cwt.cpp:172-204: a sine at the bank's middle
+    frequency through the forward CWT and the L1 inverse comes back within
+    0.25 relative L2."""
+    psi = golden["cwt512_psi"]
+    rows, nb = psi.shape
+    stride = (nb + 15) & ~15
+    tab = np.zeros((rows, stride), np.float32)
+    tab[:, :nb] = psi
+    freqs = golden["cwt512_freqs"]
+    f = freqs[freqs.size // 2]
+    x = np.sin(2 * np.pi * f * np.arange(512)).astype(np.float32)
+    w = O.cwt_real32(x, tab, stride)
+    y = O.icwt_l1_f32(w, rows - 1, True, int(golden["cwt512_voices"][0]),
+                      golden["cwt512_report"][11])
+    assert rel_l2(y, x) <= 0.25
+
+
 @pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference tree absent)")
 @pytest.mark.parametrize("n", [64, 360, 4800, 1009, 16384, 65536])
 def test_oracle_matches_live_reference(n):
