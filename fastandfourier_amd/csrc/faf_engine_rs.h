@@ -137,6 +137,93 @@ FAF_DEV void fft32768(float2 *v, float *lds, const float2 *tw, const float2 *b1,
 }  // namespace rs
 
 /* ------------------------------------------------------------------ */
+/* 8192 points in one pass: 256 threads x 32 values, four workgroups    */
+/* per CU (35 KB of LDS each).  8192 = 8 * 32 * 32 in natural order:    */
+/*   pass 1  thread t: for n1 = t + 256 i (i < 4) a DFT-8 over n2 of     */
+/*           z[n1 + 1024 n2], times W_8192^(n1 k2)                       */
+/*   pass 2  thread (k2, n1a), tid = k2 + 8 n1a: DFT-32 over n1b of     */
+/*           Y[n1a + 32 n1b][k2], times W_1024^(n1a ka)                 */
+/*   pass 3  thread (k2, ka), tid = k2 + 8 ka: DFT-32 over n1a, giving  */
+/*           X[k2 + 8 (ka + 32 kb)] = X[tid + 256 kb]                    */
+/* The two LDS exchanges go one plane at a time with row pitches 1032  */
+/* (k2 rows of Y) and 1096 / 33 (k2 blocks, ka rows): every wave access */
+/* of either touches 64 distinct banks.                                 */
+/* ------------------------------------------------------------------ */
+namespace rs8 {
+
+constexpr int NC = 8192;
+constexpr int T = 256;
+constexpr int P1 = 1032;
+constexpr int P2 = 1096;
+constexpr int Q = 33;
+constexpr int LDS_BYTES = 8 * P2 * 4;
+
+/* twiddle bases of pass 1: W^(n1 2^e), e < 3, for the four n1 of a thread */
+FAF_DEV void tw1_bases(const float2 *tw, int tid, float2 (*b)[3]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int n1 = tid + 256 * i;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) b[i][e] = tw[n1 << e];
+    }
+}
+
+/* value r of pass 1 (r = i + 4 k2 after it) goes to Y row k2, column
+ * n1 = tid + 256 i; value r of pass 2 (r = ka) to block k2, row ka */
+struct WoY { static constexpr int at(int r) { return (r >> 2) * P1 + (r & 3) * 256; } };
+struct RoY { static constexpr int at(int r) { return 32 * r; } };
+struct WoE { static constexpr int at(int r) { return r * Q; } };
+struct RoE { static constexpr int at(int r) { return r; } };
+
+template <class WO, class RO> FAF_DEV void xchg(float2 *v, float *lds, int wb, int rb) {
+#pragma unroll
+    for (int r = 0; r < 32; ++r) lds[wb + WO::at(r)] = v[r].x;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 32; ++r) v[r].x = lds[rb + RO::at(r)];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 32; ++r) lds[wb + WO::at(r)] = v[r].y;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 32; ++r) v[r].y = lds[rb + RO::at(r)];
+}
+
+/* In: v[m] = z[tid + 256 m], b1 from tw1_bases.  Out: v[j] = Z[tid + 256 j].
+ * Ends with LDS reads outstanding in other waves: barrier before reusing lds. */
+template <bool INV>
+FAF_DEV void fft8192(float2 *v, float *lds, const float2 *tw, const float2 (*b1)[3], int tid) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float2 g[8];
+#pragma unroll
+        for (int n2 = 0; n2 < 8; ++n2) g[n2] = v[i + 4 * n2];
+        dft<8, INV>(g);
+        float2 w[8];
+#pragma unroll
+        for (int k = 1; k < 8; ++k) {
+            const int e = __builtin_ctz(k);
+            w[k] = k == (1 << e) ? b1[i][e] : cmul(w[k & (k - 1)], b1[i][e]);
+            g[k] = cmul(g[k], w[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[i + 4 * k] = g[k];
+    }
+    const int k2 = tid & 7, n1a = tid >> 3;
+    float2 b2[5];
+#pragma unroll
+    for (int e = 0; e < 5; ++e) b2[e] = tw[(8 * n1a) << e];
+    xchg<WoY, RoY>(v, lds, tid, k2 * P1 + n1a);
+    rs::dft32<INV>(v);
+    rs::tw_apply(v, b2);
+    __syncthreads();
+    xchg<WoE, RoE>(v, lds, k2 * P2 + n1a, k2 * P2 + n1a * Q);
+    rs::dft32<INV>(v);
+}
+
+}  // namespace rs8
+
+/* ------------------------------------------------------------------ */
 /* kernels: persistent, one workgroup per CU, signals s = blockIdx.x,   */
 /* blockIdx.x + gridDim.x, ...                                          */
 /* ------------------------------------------------------------------ */
@@ -289,6 +376,99 @@ __global__ __launch_bounds__(1024) void kr_c2c(const float2 *tw, const float *in
             else { ore[k] = v[j].x * scale; oim[k] = v[j].y * scale; }
         }
     }
+}
+
+/* rfft N = 16384 in one pass, one workgroup per signal: x real (8-byte
+ * aligned rows), N/2+1 bins out.  The Kiss post-pass is kr_r2c's with 256
+ * threads: bin k = tid + 256 j has twiddle -i W_16384^k = post[tid] W_64^j. */
+template <bool INTER_OUT>
+__global__ __launch_bounds__(256) void kr8_r2c(const float2 *tw, const float2 *post,
+                                               const float *x, long in_dist, float *out_re,
+                                               float *out_im, long out_dist, float scale) {
+    using namespace rs8;
+    float *lds = (float *)faf_smem;
+    float2 *q = faf_smem;                         /* 16 x 256 float2 for the post-pass */
+    const int tid = (int)threadIdx.x;
+    const long s = blockIdx.x;
+    const int pt = (T - tid) & (T - 1);
+    float2 v[32], b1[4][3];
+    const float2 *z = (const float2 *)(x + s * in_dist);
+#pragma unroll
+    for (int m = 0; m < 32; ++m) v[m] = z[tid + T * m];
+    tw1_bases(tw, tid, b1);
+    const float2 pb = tid ? post[tid] : make_float2(0.0f, -1.0f);
+    fft8192<false>(v, lds, tw, b1, tid);
+    float *ore = out_re + s * out_dist, *oim = out_im + s * out_dist;
+    float2 *oi = (float2 *)out_re + s * out_dist;
+    auto put = [&](int k, float2 xk) {
+        if (INTER_OUT) oi[k] = cscale(xk, scale);
+        else { ore[k] = xk.x * scale; oim[k] = xk.y * scale; }
+    };
+    __syncthreads();
+#pragma unroll
+    for (int j = 16; j < 32; ++j) q[(j - 16) * T + tid] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        float2 xk, xm;
+        if (tid == 0 && j == 0) {
+            put(0, make_float2(v[0].x + v[0].y, 0.0f));
+            put(NC, make_float2(v[0].x - v[0].y, 0.0f));
+            const float2 h = q[0];                /* Z[4096], its own mirror */
+            kiss_post(h, h, cmul(pb, rs::w64<false>(16)), xk, xm);
+            put(NC / 2, xk);
+            continue;
+        }
+        kiss_post(v[j], q[tid ? (15 - j) * T + pt : (16 - j) * T],
+                  cmul(pb, rs::w64<false>(j)), xk, xm);
+        put(tid + T * j, xk);
+        put(tid ? pt + T * (31 - j) : T * (32 - j), xm);
+    }
+}
+
+/* irfft N = 16384 in one pass: N/2+1 bins in, y real (8-byte aligned
+ * rows); the Kiss pre-pass is kr_c2r's with 256 threads */
+template <bool INTER_IN>
+__global__ __launch_bounds__(256) void kr8_c2r(const float2 *tw, const float2 *pre,
+                                               const float *in_re, const float *in_im,
+                                               long in_dist, float *y, long out_dist,
+                                               float scale) {
+    using namespace rs8;
+    float *lds = (float *)faf_smem;
+    float2 *q = faf_smem;
+    const int tid = (int)threadIdx.x;
+    const long s = blockIdx.x;
+    const int pt = (T - tid) & (T - 1);
+    const float2 pb = tid ? pre[tid] : make_float2(0.0f, 1.0f);
+    float2 v[32], b1[4][3];
+    const float *ire = in_re + s * in_dist, *iim = in_im + s * in_dist;
+    const float2 *ii = (const float2 *)in_re + s * in_dist;
+    auto get = [&](int k) {
+        return INTER_IN ? ii[k] : make_float2(ire[k], iim[k]);
+    };
+    tw1_bases(tw, tid, b1);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const float2 a = get(tid + T * j);
+        const float2 b = get(tid ? pt + T * (31 - j) : T * (32 - j));
+        float2 fm;
+        if (tid == 0 && j == 0) {
+            v[0] = make_float2(a.x + b.x, a.x - b.x);
+            const float2 h = get(NC / 2);         /* X[4096], its own mirror */
+            kiss_pre(h, h, cmul(pb, rs::w64<true>(16)), q[0], fm);
+            continue;
+        }
+        kiss_pre(a, b, cmul(pb, rs::w64<true>(j)), v[j], fm);
+        q[tid ? (15 - j) * T + pt : (16 - j) * T] = fm;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 16; j < 32; ++j) v[j] = q[(j - 16) * T + tid];
+    __syncthreads();
+    fft8192<true>(v, lds, tw, b1, tid);
+    float2 *yo = (float2 *)(y + s * out_dist);
+#pragma unroll
+    for (int j = 0; j < 32; ++j) yo[tid + T * j] = cscale(v[j], scale);
 }
 
 #endif

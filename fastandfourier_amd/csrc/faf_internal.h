@@ -65,7 +65,8 @@ typedef struct fafh_plan {
     int spb;
     int ct;                   /* use the compile-time specialised kernel */
     int ct4;                  /* four-step with specialised column/row passes */
-    int rs;                   /* single-pass 32768-point kernel (register-resident) */
+    int rs;                   /* single-pass register-resident kernels: 32768 points
+                                 (four-step plans) or 8192 (rfft / irfft 16384) */
     float scale;
     int interleaved;          /* buffers are interleaved complex */
     void *ws;                 /* device workspace */

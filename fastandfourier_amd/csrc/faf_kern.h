@@ -36,6 +36,14 @@ int fafk_device_info(int *cus, size_t *lds_per_block, size_t *total_mem);
 const char *fafk_rs_last_error(void);
 int fafk_rs_init(void);
 int fafk_rs_pair_ok(const void *p, long dist);
+/* rfft / irfft N = 16384 in one pass: tw = W_8192 table of the direction,
+ * post / pre = Kiss tables of 8192; rows must be 8-byte aligned */
+int fafk_rs8_r2c(int inter_out, const void *tw, const void *post, const float *x, long in_dist,
+                 float *out_re, float *out_im, long out_dist, long count, float scale,
+                 void *stream);
+int fafk_rs8_c2r(int inter_in, const void *tw, const void *pre, const float *in_re,
+                 const float *in_im, long in_dist, float *y, long out_dist, long count,
+                 float scale, void *stream);
 int fafk_rs_r2c(int inter_out, const void *tw, const void *post, const float *x, long in_dist,
                 float *out_re, float *out_im, long out_dist, long count, float scale, int grid,
                 void *stream);

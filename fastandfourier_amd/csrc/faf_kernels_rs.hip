@@ -1,6 +1,7 @@
 /*
  * faf_kernels_rs.hip -- C-ABI launchers of the single-pass 32768-point
- * kernels (faf_engine_rs.h): rfft / irfft N = 65536 and C2C 32768.
+ * kernels (faf_engine_rs.h): rfft / irfft N = 65536 and C2C 32768, and rfft /
+ * irfft N = 16384.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -31,6 +32,40 @@ extern "C" int fafk_rs_init(void) {
     rs_attr(kr_c2c<false, false, false>); rs_attr(kr_c2c<true, false, false>);
     rs_attr(kr_c2c<false, true, true>); rs_attr(kr_c2c<true, true, true>);
     return 0;
+}
+
+/* rfft / irfft N = 16384 in one pass (rs8): one 256-thread workgroup per
+ * signal, grid = count */
+extern "C" int fafk_rs8_r2c(int inter_out, const void *tw, const void *post, const float *x,
+                            long in_dist, float *out_re, float *out_im, long out_dist,
+                            long count, float scale, void *stream) {
+    if (count <= 0) return 0;
+    if (count > 0x7fffffffL) return rs_fail("kr8_r2c", hipErrorInvalidValue);
+    const dim3 g((unsigned)count), b(rs8::T);
+    if (inter_out)
+        hipLaunchKernelGGL(kr8_r2c<true>, g, b, rs8::LDS_BYTES, S(stream), (const float2 *)tw,
+                           (const float2 *)post, x, in_dist, out_re, out_im, out_dist, scale);
+    else
+        hipLaunchKernelGGL(kr8_r2c<false>, g, b, rs8::LDS_BYTES, S(stream), (const float2 *)tw,
+                           (const float2 *)post, x, in_dist, out_re, out_im, out_dist, scale);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : rs_fail("kr8_r2c", e);
+}
+
+extern "C" int fafk_rs8_c2r(int inter_in, const void *tw, const void *pre, const float *in_re,
+                            const float *in_im, long in_dist, float *y, long out_dist,
+                            long count, float scale, void *stream) {
+    if (count <= 0) return 0;
+    if (count > 0x7fffffffL) return rs_fail("kr8_c2r", hipErrorInvalidValue);
+    const dim3 g((unsigned)count), b(rs8::T);
+    if (inter_in)
+        hipLaunchKernelGGL(kr8_c2r<true>, g, b, rs8::LDS_BYTES, S(stream), (const float2 *)tw,
+                           (const float2 *)pre, in_re, in_im, in_dist, y, out_dist, scale);
+    else
+        hipLaunchKernelGGL(kr8_c2r<false>, g, b, rs8::LDS_BYTES, S(stream), (const float2 *)tw,
+                           (const float2 *)pre, in_re, in_im, in_dist, y, out_dist, scale);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : rs_fail("kr8_c2r", e);
 }
 
 /* float2 row access needs 8-byte aligned rows */

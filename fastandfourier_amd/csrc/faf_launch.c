@@ -133,6 +133,9 @@ int fafi_run_c2c(fafh_plan *p, const float *ir, const float *ii, long idist, flo
 int fafi_run_r2c(fafh_plan *p, const float *x, long idist, float *orr, float *oi, long odist,
                  long count, void *stream) {
     const int inter = p->interleaved;
+    if (p->kind == PK_R2C && p->rs && fafk_rs_pair_ok(x, idist))   /* N = 16384, one pass */
+        return krerr(fafk_rs8_r2c(inter, p->a.tw, p->post, x, idist, orr, oi, odist, count,
+                                  p->scale, stream));
     if (p->kind == PK_R2C && p->ct)
         return kcerr(fafk_ct_r2c(&p->a, inter, p->post, x, idist, orr, oi, odist, count,
                                  p->scale, stream));
@@ -184,6 +187,9 @@ int fafi_run_r2c(fafh_plan *p, const float *x, long idist, float *orr, float *oi
 int fafi_run_c2r(fafh_plan *p, const float *ir, const float *ii, long idist, float *y,
                  long odist, long count, void *stream) {
     const int inter = p->interleaved;
+    if (p->kind == PK_C2R && p->rs && fafk_rs_pair_ok(y, odist))   /* N = 16384, one pass */
+        return krerr(fafk_rs8_c2r(inter, p->a.tw, p->pre, ir, ii, idist, y, odist, count,
+                                  p->scale, stream));
     if (p->kind == PK_C2R && p->ct)
         return kcerr(fafk_ct_c2r(&p->a, inter, p->pre, ir, ii, idist, y, odist, count,
                                  p->scale, stream));

@@ -341,15 +341,16 @@ def test_single_pass_32768_many_signals():
         G.destroy(t)
 
 
-def test_rfft_65536_unaligned_rows_fall_back():
-    """Rows at an odd float distance are not 8-byte aligned, so the rfft
-    takes the four-step path instead of the single pass; both agree with
-    FP64."""
-    say("case: rfft 65536 odd row distance")
+@pytest.mark.parametrize("n", [16384, 65536])
+def test_rfft_unaligned_rows_fall_back(n):
+    """Rows at an odd float distance are not 8-byte aligned, so the rfft and
+    irfft leave the single pass for the staged (16384) or four-step (65536)
+    path; both paths agree with FP64."""
+    say(f"case: rfft {n} odd row distance")
     lib = G.L()
-    n, cnt = 65536, 3
+    cnt = 3
     nb = n // 2 + 1
-    rng = np.random.default_rng(65537)
+    rng = np.random.default_rng(n + 1)
     xs = rng.standard_normal((cnt, n + 1)).astype(np.float32)
     t = G.create("rfft", n)
     try:
@@ -363,6 +364,16 @@ def test_rfft_65536_unaligned_rows_fall_back():
         truth = np.stack([O.fft64(r[:n].astype(np.float64))[0][:nb] for r in xs])
         assert rel_l2(X, truth) < tol(n)
         assert rel_l2(G.r2c(t, np.ascontiguousarray(xs[:, :n])), truth) < tol(n)
+        # the inverse into rows at an odd distance
+        ti = G.create("rfft", n, dir="inverse")
+        try:
+            y = G.empty((cnt, n + 1))
+            torch.cuda.synchronize()
+            by = fa.buffer_real(y.data_ptr(), n)
+            fa.check(lib.faf_execute_batch(ti, C.byref(by), C.byref(bo), cnt, 0, n + 1), "irfft")
+            assert rel_l2(G.host(y)[:, :n], xs[:, :n]) < tol(n)
+        finally:
+            G.destroy(ti)
     finally:
         G.destroy(t)
 
