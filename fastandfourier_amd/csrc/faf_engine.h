@@ -467,6 +467,17 @@ __global__ __launch_bounds__(MT) void k_c2r(const fafd_fft *Pd, int spb, const f
 /* op semantics src/chirp.c:421-471, src/faf_core.c:150-166)            */
 /* ------------------------------------------------------------------ */
 
+/* the op list that is exactly one MUL by the bound spectrum (the notch and
+ * filter pipelines): apply_ops without the per-bin interpretation */
+FAF_DEV float2 op_mul(const fafd_ops &ops, int k, float2 v) {
+    const long m = ops.h_bins;
+    if (k < m) {
+        if (k == 0 || k == m - 1) v = make_float2(v.x * ops.h_re[k], 0.0f);
+        else v = cmul(v, make_float2(ops.h_re[k], ops.h_im[k]));
+    }
+    return v;
+}
+
 FAF_DEV float2 apply_ops(const fafd_ops &ops, int k, float2 v) {
     for (int o = 0; o < ops.nops; ++o) {
         const int c = ops.code[o];

@@ -405,16 +405,6 @@ void kc_c2rw(fafd_fft P, const float2 *pre, const float *in_re, const float *in_
     ch.run_lazy(lds, lds, dst, lt, 1, (const float2 *)P.tw);
 }
 
-/* the op list that is exactly one MUL by the bound spectrum (the notch and
- * filter pipelines): apply_ops without the per-bin interpretation */
-FAF_DEV float2 op_mul(const fafd_ops &ops, int k, float2 v) {
-    const long m = ops.h_bins;
-    if (k < m) {
-        if (k == 0 || k == m - 1) v = make_float2(v.x * ops.h_re[k], 0.0f);
-        else v = cmul(v, make_float2(ops.h_re[k], ops.h_im[k]));
-    }
-    return v;
-}
 
 /* MUL1: ops is one MUL (op_mul); otherwise the general apply_ops */
 template <int NC, bool MUL1>

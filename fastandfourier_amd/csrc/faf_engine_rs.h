@@ -378,6 +378,102 @@ __global__ __launch_bounds__(1024) void kr_c2c(const float2 *tw, const float *in
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* 2048 points in one pass: one wave (64 threads x 32 values) per      */
+/* signal.  2048 = 8 * 16 * 16 in natural order:                       */
+/*   pass 1  thread t: for n1 = t + 64 i (i < 4) a DFT-8 over n2 of     */
+/*           z[n1 + 256 n2], times W_2048^(n1 k2)                        */
+/*   pass 2  thread (k2, a), tid = k2 + 8 a: for n1a = a, a + 8 a       */
+/*           DFT-16 over n1b of Y[n1a + 16 n1b][k2], times W_256^(n1a ka) */
+/*   pass 3  thread (k2, b), tid = k2 + 8 b: for ka = b, b + 8 a DFT-16  */
+/*           over n1a, giving X[k2 + 8 (ka + 16 kb)] = X[tid + 64 j],    */
+/*           j = (ka >> 3) + 2 kb                                       */
+/* Exchanges one plane at a time: Y rows of pitch 280, then blocks of  */
+/* 280 floats with ka rows of pitch 17; every wave access touches 64   */
+/* distinct banks.  2240 floats of LDS per signal.                      */
+/* ------------------------------------------------------------------ */
+namespace rs2 {
+
+constexpr int NC = 2048;
+constexpr int T = 64;
+constexpr int P1 = 280;
+constexpr int P2 = 280;
+constexpr int Q = 17;
+constexpr int LDS_BYTES = 8 * P1 * 4;
+
+struct WoY { static constexpr int at(int r) { return (r >> 2) * P1 + (r & 3) * 64; } };
+struct RoY { static constexpr int at(int r) { return 8 * (r >> 4) + 16 * (r & 15); } };
+struct WoE { static constexpr int at(int r) { return (r & 15) * Q + 8 * (r >> 4); } };
+struct RoE { static constexpr int at(int r) { return 8 * Q * (r >> 4) + (r & 15); } };
+
+/* twiddle bases of pass 1: W^(n1 2^e), e < 3, for the four n1 of a thread */
+FAF_DEV void tw1_bases(const float2 *tw, int tid, float2 (*b)[3]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) b[i][e] = tw[(tid + 64 * i) << e];
+    }
+}
+
+/* v[k] *= W^(s k), k = 1..15, from the bases W^(s 2^e), e < 4 */
+FAF_DEV void tw_apply16(float2 *v, const float2 *b) {
+    float2 w[16];
+#pragma unroll
+    for (int k = 1; k < 16; ++k) {
+        const int e = __builtin_ctz(k);
+        w[k] = k == (1 << e) ? b[e] : cmul(w[k & (k - 1)], b[e]);
+        v[k] = cmul(v[k], w[k]);
+    }
+}
+
+/* In: v[m] = z[tid + 64 m], b1 from tw1_bases.  Out: v[j] = Z[tid + 64 j].
+ * Ends with LDS reads outstanding: barrier before reusing lds. */
+template <bool INV>
+FAF_DEV void fft2048(float2 *v, float *lds, const float2 *tw, const float2 (*b1)[3], int tid) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float2 g[8];
+#pragma unroll
+        for (int n2 = 0; n2 < 8; ++n2) g[n2] = v[i + 4 * n2];
+        dft<8, INV>(g);
+        float2 w[8];
+#pragma unroll
+        for (int k = 1; k < 8; ++k) {
+            const int e = __builtin_ctz(k);
+            w[k] = k == (1 << e) ? b1[i][e] : cmul(w[k & (k - 1)], b1[i][e]);
+            g[k] = cmul(g[k], w[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[i + 4 * k] = g[k];
+    }
+    const int k2 = tid & 7, a = tid >> 3;
+    float2 b2[2][4];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) b2[h][e] = tw[(8 * (a + 8 * h)) << e];
+    }
+    rs8::xchg<WoY, RoY>(v, lds, tid, k2 * P1 + a);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        dft16<INV>(v + 16 * h);
+        tw_apply16(v + 16 * h, b2[h]);
+    }
+    __syncthreads();
+    rs8::xchg<WoE, RoE>(v, lds, k2 * P2 + a, k2 * P2 + a * Q);
+    float2 u[32];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        dft16<INV>(v + 16 * h);
+#pragma unroll
+        for (int kb = 0; kb < 16; ++kb) u[h + 2 * kb] = v[16 * h + kb];
+    }
+#pragma unroll
+    for (int j = 0; j < 32; ++j) v[j] = u[j];
+}
+
+}  // namespace rs2
+
 /* rfft N = 16384 in one pass, one workgroup per signal: x real (8-byte
  * aligned rows), N/2+1 bins out.  The Kiss post-pass is kr_r2c's with 256
  * threads: bin k = tid + 256 j has twiddle -i W_16384^k = post[tid] W_64^j. */
@@ -466,6 +562,114 @@ __global__ __launch_bounds__(256) void kr8_c2r(const float2 *tw, const float2 *p
     for (int j = 16; j < 32; ++j) v[j] = q[(j - 16) * T + tid];
     __syncthreads();
     fft8192<true>(v, lds, tw, b1, tid);
+    float2 *yo = (float2 *)(y + s * out_dist);
+#pragma unroll
+    for (int j = 0; j < 32; ++j) yo[tid + T * j] = cscale(v[j], scale);
+}
+
+/* irfft N = 4096 in one pass, one wave per signal */
+template <bool INTER_IN>
+__global__ __launch_bounds__(64) void kr2_c2r(const float2 *tw, const float2 *pre,
+                                              const float *in_re, const float *in_im,
+                                              long in_dist, float *y, long out_dist,
+                                              float scale) {
+    using namespace rs2;
+    float *lds = (float *)faf_smem;
+    float2 *q = faf_smem;
+    const int tid = (int)threadIdx.x;
+    const long s = blockIdx.x;
+    const int pt = (T - tid) & (T - 1);
+    const float2 pb = tid ? pre[tid] : make_float2(0.0f, 1.0f);
+    float2 v[32], b1[4][3];
+    const float *ire = in_re + s * in_dist, *iim = in_im + s * in_dist;
+    const float2 *ii = (const float2 *)in_re + s * in_dist;
+    auto get = [&](int k) {
+        return INTER_IN ? ii[k] : make_float2(ire[k], iim[k]);
+    };
+    tw1_bases(tw, tid, b1);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const float2 a = get(tid + T * j);
+        const float2 b = get(tid ? pt + T * (31 - j) : T * (32 - j));
+        float2 fm;
+        if (tid == 0 && j == 0) {
+            v[0] = make_float2(a.x + b.x, a.x - b.x);
+            const float2 h = get(NC / 2);         /* X[1024], its own mirror */
+            kiss_pre(h, h, cmul(pb, rs::w64<true>(16)), q[0], fm);
+            continue;
+        }
+        kiss_pre(a, b, cmul(pb, rs::w64<true>(j)), v[j], fm);
+        q[tid ? (15 - j) * T + pt : (16 - j) * T] = fm;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 16; j < 32; ++j) v[j] = q[(j - 16) * T + tid];
+    __syncthreads();
+    fft2048<true>(v, lds, tw, b1, tid);
+    float2 *yo = (float2 *)(y + s * out_dist);
+#pragma unroll
+    for (int j = 0; j < 32; ++j) yo[tid + T * j] = cscale(v[j], scale);
+}
+
+/* The single-MUL spectral pipeline at N = 4096 (the notch) in one pass, one
+ * wave per signal: rfft, X[k] := op_mul(X[k]) on every bin, irfft.  After
+ * the forward FFT each thread holds Z[k], k = tid + 64 j; for its pairs
+ * (k, 2048 - k), j < 16, it forms X[k] and X[2048 - k] (Kiss post), applies
+ * the MUL to both, and forms F[k] and F[2048 - k] (Kiss pre).  F[2048 - k]
+ * goes back into the LDS slot its partner value came from, so the partner
+ * finds it there; the inverse FFT then runs on F. */
+__global__ __launch_bounds__(64) void kr2_pipe(const float2 *twf, const float2 *twi,
+                                               const float2 *post, const float2 *pre,
+                                               fafd_ops ops, const float *x, long in_dist,
+                                               float *y, long out_dist, float scale) {
+    using namespace rs2;
+    float *lds = (float *)faf_smem;
+    float2 *q = faf_smem;                         /* 16 x 64 float2 for the bin pairs */
+    const int tid = (int)threadIdx.x;
+    const long s = blockIdx.x;
+    const int pt = (T - tid) & (T - 1);
+    float2 v[32], b1[4][3];
+    const float2 *z = (const float2 *)(x + s * in_dist);
+#pragma unroll
+    for (int m = 0; m < 32; ++m) v[m] = z[tid + T * m];
+    tw1_bases(twf, tid, b1);
+    const float2 pb = tid ? post[tid] : make_float2(0.0f, -1.0f);
+    const float2 qb = tid ? pre[tid] : make_float2(0.0f, 1.0f);
+    fft2048<false>(v, lds, twf, b1, tid);
+    __syncthreads();
+#pragma unroll
+    for (int j = 16; j < 32; ++j) q[(j - 16) * T + tid] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        float2 xk, xm;
+        if (tid == 0 && j == 0) {
+            xk = op_mul(ops, 0, make_float2(v[0].x + v[0].y, 0.0f));
+            xm = op_mul(ops, NC, make_float2(v[0].x - v[0].y, 0.0f));
+            v[0] = make_float2(xk.x + xm.x, xk.x - xm.x);
+            const float2 h = q[0];                /* Z[1024], its own mirror */
+            float2 fm;
+            kiss_post(h, h, cmul(pb, rs::w64<false>(16)), xk, xm);
+            xk = op_mul(ops, NC / 2, xk);
+            kiss_pre(xk, xk, cmul(qb, rs::w64<true>(16)), q[0], fm);
+            continue;
+        }
+        const int slot = tid ? (15 - j) * T + pt : (16 - j) * T;
+        const int k = tid + T * j, m = NC - k;
+        kiss_post(v[j], q[slot], cmul(pb, rs::w64<false>(j)), xk, xm);
+        xk = op_mul(ops, k, xk);
+        xm = op_mul(ops, m, xm);
+        kiss_pre(xk, xm, cmul(qb, rs::w64<true>(j)), v[j], q[slot]);
+        /* keep the spectrum loads of later bins from being hoisted into
+         * registers all at once */
+        if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+    tw1_bases(twi, tid, b1);
+    __syncthreads();
+#pragma unroll
+    for (int j = 16; j < 32; ++j) v[j] = q[(j - 16) * T + tid];
+    __syncthreads();
+    fft2048<true>(v, lds, twi, b1, tid);
     float2 *yo = (float2 *)(y + s * out_dist);
 #pragma unroll
     for (int j = 0; j < 32; ++j) yo[tid + T * j] = cscale(v[j], scale);

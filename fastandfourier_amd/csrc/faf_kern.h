@@ -38,6 +38,16 @@ int fafk_rs_init(void);
 int fafk_rs_pair_ok(const void *p, long dist);
 /* rfft / irfft N = 16384 in one pass: tw = W_8192 table of the direction,
  * post / pre = Kiss tables of 8192; rows must be 8-byte aligned */
+/* irfft N = 4096 in one pass: tw = inverse W_2048 table, pre = inverse
+ * Kiss table of 2048; rows must be 8-byte aligned */
+/* the single-MUL spectral pipeline N = 4096 in one pass: twf / twi = forward
+ * / inverse W_2048 tables; x and y rows must be 8-byte aligned */
+int fafk_rs2_pipe(const void *twf, const void *twi, const void *post, const void *pre,
+                  const fafd_ops *ops, const float *x, long in_dist, float *y, long out_dist,
+                  long count, float scale, void *stream);
+int fafk_rs2_c2r(int inter_in, const void *tw, const void *pre, const float *in_re,
+                 const float *in_im, long in_dist, float *y, long out_dist, long count,
+                 float scale, void *stream);
 int fafk_rs8_r2c(int inter_out, const void *tw, const void *post, const float *x, long in_dist,
                  float *out_re, float *out_im, long out_dist, long count, float scale,
                  void *stream);

@@ -1,7 +1,7 @@
 /*
  * faf_kernels_rs.hip -- C-ABI launchers of the single-pass 32768-point
- * kernels (faf_engine_rs.h): rfft / irfft N = 65536 and C2C 32768, and rfft /
- * irfft N = 16384.
+ * kernels (faf_engine_rs.h): rfft / irfft N = 65536 and C2C 32768, rfft /
+ * irfft N = 16384, and irfft N = 4096 with the single-MUL pipeline at 4096.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -66,6 +66,36 @@ extern "C" int fafk_rs8_c2r(int inter_in, const void *tw, const void *pre, const
                            (const float2 *)pre, in_re, in_im, in_dist, y, out_dist, scale);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : rs_fail("kr8_c2r", e);
+}
+
+/* irfft N = 4096 in one pass (rs2): one wave per signal */
+extern "C" int fafk_rs2_c2r(int inter_in, const void *tw, const void *pre, const float *in_re,
+                            const float *in_im, long in_dist, float *y, long out_dist,
+                            long count, float scale, void *stream) {
+    if (count <= 0) return 0;
+    if (count > 0x7fffffffL) return rs_fail("kr2_c2r", hipErrorInvalidValue);
+    const dim3 g((unsigned)count), b(rs2::T);
+    if (inter_in)
+        hipLaunchKernelGGL(kr2_c2r<true>, g, b, rs2::LDS_BYTES, S(stream), (const float2 *)tw,
+                           (const float2 *)pre, in_re, in_im, in_dist, y, out_dist, scale);
+    else
+        hipLaunchKernelGGL(kr2_c2r<false>, g, b, rs2::LDS_BYTES, S(stream), (const float2 *)tw,
+                           (const float2 *)pre, in_re, in_im, in_dist, y, out_dist, scale);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : rs_fail("kr2_c2r", e);
+}
+
+/* the single-MUL pipeline N = 4096 in one pass (rs2): one wave per signal */
+extern "C" int fafk_rs2_pipe(const void *twf, const void *twi, const void *post, const void *pre,
+                             const fafd_ops *ops, const float *x, long in_dist, float *y,
+                             long out_dist, long count, float scale, void *stream) {
+    if (count <= 0) return 0;
+    if (count > 0x7fffffffL) return rs_fail("kr2_pipe", hipErrorInvalidValue);
+    hipLaunchKernelGGL(kr2_pipe, dim3((unsigned)count), dim3(rs2::T), rs2::LDS_BYTES, S(stream),
+                       (const float2 *)twf, (const float2 *)twi, (const float2 *)post,
+                       (const float2 *)pre, *ops, x, in_dist, y, out_dist, scale);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : rs_fail("kr2_pipe", e);
 }
 
 /* float2 row access needs 8-byte aligned rows */

@@ -187,9 +187,13 @@ int fafi_run_r2c(fafh_plan *p, const float *x, long idist, float *orr, float *oi
 int fafi_run_c2r(fafh_plan *p, const float *ir, const float *ii, long idist, float *y,
                  long odist, long count, void *stream) {
     const int inter = p->interleaved;
-    if (p->kind == PK_C2R && p->rs && fafk_rs_pair_ok(y, odist))   /* N = 16384, one pass */
+    if (p->kind == PK_C2R && p->rs && fafk_rs_pair_ok(y, odist)) {   /* one pass */
+        if (p->n == 4096)
+            return krerr(fafk_rs2_c2r(inter, p->a.tw, p->pre, ir, ii, idist, y, odist,
+                                      count, p->scale, stream));
         return krerr(fafk_rs8_c2r(inter, p->a.tw, p->pre, ir, ii, idist, y, odist, count,
                                   p->scale, stream));
+    }
     if (p->kind == PK_C2R && p->ct)
         return kcerr(fafk_ct_c2r(&p->a, inter, p->pre, ir, ii, idist, y, odist, count,
                                  p->scale, stream));
@@ -246,6 +250,10 @@ int fafi_run_pipe(fafh_plan *p, const float *x, long idist, float *y, long odist
         if (ops.code[i] == FAFD_OP_MUL && !ops.h_re) {
             ops.code[i] = FAFD_OP_NONE;   /* unbound spectrum: no-op, as the reference */
         }
+    if (p->ct && p->n == 4096 && ops.nops == 1 && ops.code[0] == FAFD_OP_MUL &&
+        fafk_rs_pair_ok(x, idist) && fafk_rs_pair_ok(y, odist))   /* one pass */
+        return krerr(fafk_rs2_pipe(p->a.tw, p->b.tw, p->post, p->pre, &ops, x, idist, y, odist,
+                                   count, p->scale, stream));
     if (p->ct)
         return kcerr(fafk_ct_pipe(&p->a, &p->b, p->post, p->pre, &ops, x, idist, y, odist,
                                   count, p->scale, stream));

@@ -573,8 +573,10 @@ int fafi_plan_real(fafh_plan *p, size_t n, int inv) {
         if (fafi_make_fft(nc, inv, 0, NULL, 0, &p->a)) return -1;
         p->kind = inv ? PK_C2R : PK_R2C;
         p->ct = fafi_ct_enabled() && fafk_ct_supports(1, (int)nc);
-        /* N = 16384: the single-pass register-resident kernels (rs8) */
-        p->rs = fafi_ct_enabled() && nc == 8192;
+        /* the single-pass register-resident kernels: rfft / irfft 16384 and
+         * irfft 4096 (the rfft 4096 one-signal-per-workgroup kernel is faster
+         * than the single pass) */
+        p->rs = fafi_ct_enabled() && (nc == 8192 || (nc == 2048 && inv));
         int t = fafi_tuned_spb(p->kind, n);
         p->spb = t > 0 ? t : fafi_default_spb(&p->a, 0);
         return p->spb >= 1 ? 0 : -1;
