@@ -341,11 +341,11 @@ def test_single_pass_32768_many_signals():
         G.destroy(t)
 
 
-@pytest.mark.parametrize("n", [16384, 65536])
+@pytest.mark.parametrize("n", [4096, 16384, 65536])
 def test_rfft_unaligned_rows_fall_back(n):
     """Rows at an odd float distance are not 8-byte aligned, so the rfft and
-    irfft leave the single pass for the staged (16384) or four-step (65536)
-    path; both paths agree with FP64."""
+    irfft leave the single pass for the staged (4096, 16384) or four-step
+    (65536) path; both paths agree with FP64."""
     say(f"case: rfft {n} odd row distance")
     lib = G.L()
     cnt = 3
@@ -437,6 +437,40 @@ def test_pipeline_ops_match_oracle(n):
         y = G.real_to_real(t, x)
         ref = O.pipeline32(x, n, [2, 1, 3], [0, 0, 2], [0, 0, nb - 3], h)
         assert rel_l2(y, ref) < 2 * tol(n)
+    finally:
+        lib.faf_destroy_transform(t)
+
+
+@pytest.mark.parametrize("bins", [2049, 1949, 100])
+def test_single_mul_pipeline_4096(bins):
+    """The single-MUL pipeline at N=4096 (one pass) against the oracle, for a
+    complex spectrum bound over all bins and over fewer (the last bound bin
+    is then multiplied as a real Nyquist bin, reference src/faf_core.c:150-220),
+    on packed rows and on rows at an odd distance (the staged fallback)."""
+    say(f"case: single-MUL pipeline 4096, {bins} bound bins")
+    lib = G.L()
+    n = 4096
+    rng = np.random.default_rng(bins)
+    h = (rng.uniform(-1, 1, bins) + 1j * rng.uniform(-1, 1, bins)).astype(np.complex64)
+    ops = (C.c_int * 1)(2)
+    a0 = (C.c_uint32 * 1)(0)
+    t = lib.faf_create_pipeline(C.byref(fa.config_init(n)), 1, ops, a0, a0)
+    assert t, lib.faf_get_error()
+    try:
+        hr = G.dev(np.ascontiguousarray(h.real))
+        hi = G.dev(np.ascontiguousarray(h.imag))
+        assert lib.chirp_bind(t, b"H", hr.data_ptr(), hi.data_ptr(), bins) == 0
+        x = rng.standard_normal((5, n + 1)).astype(np.float32)
+        ref = O.pipeline32(np.ascontiguousarray(x[:, :n]), n, [2], [0], [0], h)
+        y = G.real_to_real(t, np.ascontiguousarray(x[:, :n]))
+        assert rel_l2(y, ref) < 2 * tol(n)
+        dx = G.dev(x)
+        dy = G.empty((5, n + 1))
+        torch.cuda.synchronize()
+        fa.check(lib.faf_execute_batch(t, C.byref(fa.buffer_real(dy.data_ptr(), n)),
+                                       C.byref(fa.buffer_real(dx.data_ptr(), n)), 5, n + 1,
+                                       n + 1), "pipeline odd rows")
+        assert rel_l2(G.host(dy)[:, :n], ref) < 2 * tol(n)
     finally:
         lib.faf_destroy_transform(t)
 
