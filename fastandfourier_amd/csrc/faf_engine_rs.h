@@ -611,6 +611,7 @@ __global__ __launch_bounds__(64) void kr2_c2r(const float2 *tw, const float2 *pr
     for (int j = 0; j < 32; ++j) yo[tid + T * j] = cscale(v[j], scale);
 }
 
+#ifndef FAF_RS_CORE_ONLY   /* set by files that use only the cores above */
 /* The single-MUL spectral pipeline at N = 4096 (the notch) in one pass, one
  * wave per signal: rfft, X[k] := op_mul(X[k]) on every bin, irfft.  After
  * the forward FFT each thread holds Z[k], k = tid + 64 j; for its pairs
@@ -674,5 +675,7 @@ __global__ __launch_bounds__(64) void kr2_pipe(const float2 *twf, const float2 *
 #pragma unroll
     for (int j = 0; j < 32; ++j) yo[tid + T * j] = cscale(v[j], scale);
 }
+
+#endif
 
 #endif

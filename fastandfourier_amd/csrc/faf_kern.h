@@ -115,6 +115,7 @@ int fafk_blue(const fafd_fft *Pf, const fafd_fft *Pi, const fafd_blue *Bq, int s
               float *out_im, long out_dist, long count, float scale, void *stream);
 int fafk_rg_init(void);   /* faf_kernels_rg.hip */
 const char *fafk_rg_last_error(void);
+int fafk_rg_ct_slots(int n);
 int fafk_rg_ct_threads(int n);
 int fafk_rg_rader_threads(int p);
 int fafk_ragged(const void *plans, const void *units, int nunits, const int32_t *order,

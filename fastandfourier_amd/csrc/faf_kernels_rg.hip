@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include "faf_engine_ct.h"
+#define FAF_RS_CORE_ONLY
+#include "faf_engine_rs.h"
 #include "faf_kern.h"
 
 static thread_local char g_rg_err[160];
@@ -80,8 +82,45 @@ FAF_DEV void rg_rader(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut
 /* GEN: also the runtime Stockham, Rader and Bluestein paths.  Without them
  * the kernel needs far fewer registers (more workgroups per CU); the host
  * picks it when every length of the batch has compile-time passes. */
+/* a pointer that is the same in every lane of the wave, in scalar registers */
+template <class P> FAF_DEV P *wave_uniform(P *p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (P *)(((uint64_t)hi << 32) | lo);
+}
+
+/* 8192 points: the single-pass register-resident FFT, 256 threads and
+ * 35 KB of LDS per signal, so a 512-thread workgroup runs two signals.  A
+ * signal's 256 threads are four whole waves, so its plane pointers are
+ * wave-uniform: in scalar registers the 64 loads and stores address by a
+ * 32-bit lane offset instead of a 64-bit address per load. */
+FAF_DEV void rg_rs8(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst, int sid,
+                    int lt, int active) {
+    float *lds = (float *)faf_smem + sid * (rs8::LDS_BYTES / 4);
+    const float2 *tw = (const float2 *)pl->fwd.tw;
+    const float *ir = wave_uniform(src.re), *ii = wave_uniform(src.im);
+    float *orr = wave_uniform(dst.re), *oi = wave_uniform(dst.im);
+    float2 v[32], b1[4][3];
+#pragma unroll
+    for (int m = 0; m < 32; ++m) {
+        const int k = lt + rs8::T * m;
+        v[m] = active ? make_float2(ir[k], ii[k]) : make_float2(0.0f, 0.0f);
+    }
+    rs8::tw1_bases(tw, lt, b1);
+    rs8::fft8192<false>(v, lds, tw, b1, lt);
+    if (active) {
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            const int k = lt + rs8::T * j;
+            orr[k] = v[j].x;
+            oi[k] = v[j].y;
+        }
+    }
+}
+
 template <int MT, bool GEN>
-__global__ __launch_bounds__(MT) void kg_ragged(const fafd_rplan *plans, const fafd_unit *units,
+__global__ __launch_bounds__(MT, GEN ? 1 : 4) void kg_ragged(const fafd_rplan *plans, const fafd_unit *units,
                                                 int nunits, const int32_t *order,
                                                 const long *offsets, const float *in_re,
                                                 const float *in_im, float *out_re,
@@ -110,7 +149,7 @@ __global__ __launch_bounds__(MT) void kg_ragged(const fafd_rplan *plans, const f
             case 1024: rg_ct<1024>(pl, src, dst, sid, lt, active); break;
             case 2048: rg_ct<2048>(pl, src, dst, sid, lt, active); break;
             case 4096: rg_ct<4096>(pl, src, dst, sid, lt, active); break;
-            case 8192: rg_ct<8192>(pl, src, dst, sid, lt, active); break;
+            case 8192: rg_rs8(pl, src, dst, sid, lt, active); break;
             case 60: rg_mix<60>(pl, src, dst, sid, lt, active); break;
             case 360: rg_mix<360>(pl, src, dst, sid, lt, active); break;
             case 1000: rg_mix<1000>(pl, src, dst, sid, lt, active); break;
@@ -182,6 +221,10 @@ extern "C" int fafk_ragged(const void *plans, const void *units, int nunits,
 
 extern "C" const char *fafk_rg_last_error(void) { return g_rg_err; }
 
+/* LDS per signal (float2 slots) of the compile-time path of length n when it
+ * is not the plan's plane, else 0 */
+extern "C" int fafk_rg_ct_slots(int n) { return n == 8192 ? rs8::LDS_BYTES / 8 : 0; }
+
 /* threads per signal of the Rader path for prime p with compile-time inner
  * passes, 0 if p has none */
 extern "C" int fafk_rg_rader_threads(int p) {
@@ -199,7 +242,8 @@ extern "C" int fafk_rg_rader_threads(int p) {
  * launch, 0 if n has none */
 extern "C" int fafk_rg_ct_threads(int n) {
     switch (n) {
-    case 256: case 512: case 1024: case 2048: case 4096: case 8192: return n / 16;
+    case 256: case 512: case 1024: case 2048: case 4096: return n / 16;
+    case 8192: return rs8::T;
     case 60: return MixSeq<60>::T;
     case 360: return MixSeq<360>::T;
     case 1000: return MixSeq<1000>::T;

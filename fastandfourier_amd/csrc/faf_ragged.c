@@ -97,6 +97,7 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
             d->fwd = p->a;
             if (ct) d->fwd.threads = ct_t;
             per = (size_t)p->a.plane;
+            if (ct && fafk_rg_ct_slots((int)n) > 0) per = (size_t)fafk_rg_ct_slots((int)n);
         } else if (p->kind == PK_RADER) {
             const int rt = fafi_ct_enabled() ? fafk_rg_rader_threads((int)uniq[k]) : 0;
             d->kind = rt > 0 ? FAFD_KIND_RADER_CT : FAFD_KIND_RADER;
