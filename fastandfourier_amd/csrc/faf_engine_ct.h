@@ -251,6 +251,49 @@ void kc_r2c(fafd_fft P, const float2 *post, const float *x, long in_dist, float 
     }
     ch.run(src, lds, lds, lt, active);
     __syncthreads();
+    /* packed split rows of a whole workgroup are one contiguous run of
+     * SPB (NC + 1) bins per plane: staged through LDS and stored as aligned
+     * float4 when the run allows it (as in kc_stft) */
+    constexpr int NB = NC + 1, RUN = SPB * NB;
+    if constexpr (!INTER_OUT && HOIST && RUN % 4 == 0 &&
+                  RUN * 2 * 4 <= SPB * (NC + NC / 16 + 1) * 8) {
+        const long g0 = (long)blockIdx.x * SPB;
+        float *re0 = out_re + g0 * NB, *im0 = out_im + g0 * NB;
+        if (out_dist == NB && g0 + SPB <= count &&
+            ((((uintptr_t)re0) | ((uintptr_t)im0)) & 15) == 0) {
+            float2 xs[NP][2];
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                const int k = lt + i * T;
+                if (k > NC / 2) break;
+                if (k == 0) {
+                    float2 z = lds.ld(0);
+                    xs[i][0] = make_float2(z.x + z.y, 0.0f);
+                    xs[i][1] = make_float2(z.x - z.y, 0.0f);
+                } else {
+                    kiss_post(lds.ld(k), lds.ld(NC - k), pw[i], xs[i][0], xs[i][1]);
+                }
+            }
+            __syncthreads();                      /* every signal is out of LDS */
+            float *sre = (float *)faf_smem, *sim = sre + RUN;
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                const int k = lt + i * T;
+                if (k > NC / 2) break;
+                sre[sid * NB + k] = xs[i][0].x * scale; sim[sid * NB + k] = xs[i][0].y * scale;
+                sre[sid * NB + NC - k] = xs[i][1].x * scale;
+                sim[sid * NB + NC - k] = xs[i][1].y * scale;
+            }
+            __syncthreads();
+            const float4 *s4r = (const float4 *)sre, *s4i = (const float4 *)sim;
+            float4 *o4r = (float4 *)re0, *o4i = (float4 *)im0;
+            for (int q = tid; q < RUN / 4; q += SPB * T) {
+                o4r[q] = s4r[q];
+                o4i[q] = s4i[q];
+            }
+            return;
+        }
+    }
     if (!active) return;
     float *ore = out_re + sig * out_dist;
     float *oim = out_im + sig * out_dist;
