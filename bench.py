@@ -225,7 +225,7 @@ def traffic_from_profile():
     """HBM bytes per launch of the headline kernel, from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy
     kernel of known size), or None when no such summary exists."""
-    p = os.path.join(ROOT, "profiles", "kc_r2cw_rfft4096_pmc.txt")
+    p = os.path.join(ROOT, "profiles", "kc_r2c_rfft4096_pmc.txt")
     if not os.path.exists(p):
         return None
     for line in open(p):

@@ -224,11 +224,13 @@ void kc_c2c(fafd_fft P, const float *in_re, const float *in_im, long in_dist, fl
     }
 }
 
-template <bool INTER_OUT, int NC>
-__global__ __launch_bounds__((Ct<false, NC>::T * ct_spb<Ct<false, NC>::T>()))
+/* SPBX > 0 overrides the signals per workgroup (rfft 4096: 4, so that a
+ * workgroup's packed rows make a run of whole aligned float4s) */
+template <bool INTER_OUT, int NC, int SPBX = 0>
+__global__ __launch_bounds__((Ct<false, NC>::T * (SPBX > 0 ? SPBX : ct_spb<Ct<false, NC>::T>())))
 void kc_r2c(fafd_fft P, const float2 *post, const float *x, long in_dist, float *out_re,
             float *out_im, long out_dist, long count, float scale) {
-    constexpr int T = Ct<false, NC>::T, SPB = ct_spb<T>();
+    constexpr int T = Ct<false, NC>::T, SPB = SPBX > 0 ? SPBX : ct_spb<T>();
     const int tid = (int)threadIdx.x, sid = tid / T, lt = tid % T;
     const long g = (long)blockIdx.x * SPB + sid;
     const int active = g < count;

@@ -85,6 +85,7 @@ extern "C" int fafk_ct_init(void) {
     ct_attrs_cwt<8192>();
     ct_attrs_n<256>(); ct_attrs_n<512>(); ct_attrs_n<1024>(); ct_attrs_n<2048>();
     ct_attrs_n<4096>(); ct_attrs_n<8192>(); ct_attrs_n<16384>();
+    ct_attr(kc_r2c<false, 2048, 4>);
     return 0;
 }
 
@@ -131,6 +132,19 @@ extern "C" int fafk_ct_r2c(const fafd_fft *P, int inter_out, const void *post, c
                            long in_dist, float *out_re, float *out_im, long out_dist,
                            long count, float scale, void *stream) {
     const float2 *po = (const float2 *)post;
+    /* rfft 4096 into split rows: four signals per 512-thread workgroup.  On
+     * packed rows with 16-byte aligned planes a whole workgroup's 4 x 2049
+     * bins per plane leave through LDS as aligned float4 stores; otherwise
+     * (tail workgroup, other distances or alignments) per bin.  Every split
+     * call takes this kernel, whatever its count and layout, so a signal's
+     * bits do not depend on the batch it is in. */
+    if (P->n == 2048 && !inter_out) {
+        hipLaunchKernelGGL((kc_r2c<false, 2048, 4>), dim3((unsigned)((count + 3) / 4)), dim3(512),
+                           (size_t)4 * P->plane * sizeof(float2), S(stream), *P, po, x, in_dist,
+                           out_re, out_im, out_dist, count, scale);
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? 0 : ct_fail("kc_r2c (4 signals)", e);
+    }
     /* one signal of NC/8 threads per workgroup where that measured faster
      * (rfft 2048, 4096, 16384; the staged kernel wins at 1024) */
     if (P->n == 1024 || P->n == 2048 || P->n == 8192) {
