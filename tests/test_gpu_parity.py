@@ -631,6 +631,67 @@ def test_framed_stft():
         G.destroy(t)
 
 
+@pytest.mark.parametrize("win, hop", [(1024, 256), (512, 128), (2048, 512)])
+def test_stft_staged_and_per_bin_stores_agree(win, hop):
+    """Whole workgroups of frames store their bins through LDS as aligned
+    float4 when the output planes are 16-byte aligned, and per bin otherwise;
+    both give the same bits, and both match FP64 (the frame count leaves a
+    partial last workgroup)."""
+    say(f"case: stft staged vs per-bin stores win={win}")
+    lib = G.L()
+    length, recs = 9 * win + 77, 3
+    t = G.create("stft", win, layout="real", win_length=win, hop_length=hop)
+    try:
+        rng = np.random.default_rng(win)
+        x = rng.standard_normal((recs, length)).astype(np.float32)
+        nf = lib.faf_stft_frames(t, length)
+        nb = win // 2 + 1
+        tot = recs * nf * nb
+        dx = G.dev(x)
+        out = {}
+        for shift in (0, 1):
+            re, im = G.empty(tot + 4), G.empty(tot + 4)
+            torch.cuda.synchronize()
+            fa.check(lib.faf_execute_stft(t, re.data_ptr() + 4 * shift, im.data_ptr() + 4 * shift,
+                                          dx.data_ptr(), length, recs, 0), "stft")
+            out[shift] = (G.host(re)[shift:shift + tot], G.host(im)[shift:shift + tot])
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+        S = (out[0][0] + 1j * out[0][1]).reshape(recs, nf, nb)
+        for r in range(recs):
+            assert rel_l2(S[r], O.stft64(x[r], win, hop)) < tol(win)
+    finally:
+        G.destroy(t)
+
+
+@pytest.mark.parametrize("n", [512, 1024, 4096])
+def test_rfft_staged_and_per_bin_stores_agree(n):
+    """The rfft kernels that stage whole workgroups of packed rows through
+    LDS (aligned float4 stores) give the same bits into unaligned planes,
+    where they store per bin, and match FP64."""
+    say(f"case: rfft staged vs per-bin stores n={n}")
+    lib = G.L()
+    cnt = 37
+    nb = n // 2 + 1
+    rng = np.random.default_rng(n + 3)
+    x = rng.standard_normal((cnt, n)).astype(np.float32)
+    t = G.create("rfft", n)
+    try:
+        dx = G.dev(x)
+        out = {}
+        for shift in (0, 1):
+            re, im = G.empty(cnt * nb + 4), G.empty(cnt * nb + 4)
+            torch.cuda.synchronize()
+            bo = fa.buffer_hermitian(re.data_ptr() + 4 * shift, im.data_ptr() + 4 * shift, nb)
+            fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(fa.buffer_real(dx.data_ptr(), n)),
+                                           cnt, 0, 0), "rfft")
+            out[shift] = (G.host(re)[shift:shift + cnt * nb], G.host(im)[shift:shift + cnt * nb])
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+        X = (out[0][0] + 1j * out[0][1]).reshape(cnt, nb)
+        assert rel_l2(X, np.fft.rfft(x.astype(np.float64), axis=1)) < tol(n)
+    finally:
+        G.destroy(t)
+
+
 def _cwt_create(n, fs, voices=None, fmin=None, fmax=None):
     lib = G.L()
     cc = lib.faf_cwt_config_init(n)
