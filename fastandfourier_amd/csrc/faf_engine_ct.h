@@ -259,23 +259,26 @@ void kc_r2c(fafd_fft P, const float2 *post, const float *x, long in_dist, float 
     constexpr int NB = NC + 1, RUN = SPB * NB;
     if constexpr (!INTER_OUT && HOIST && RUN % 4 == 0 &&
                   RUN * 2 * 4 <= SPB * (NC + NC / 16 + 1) * 8) {
+        /* the bins are formed once, by one copy of the code, and only then
+         * stored either way: a signal's bits do not depend on whether its
+         * workgroup staged */
+        float2 xs[NP][2];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const int k = lt + i * T;
+            if (k > NC / 2) break;
+            if (k == 0) {
+                float2 z = lds.ld(0);
+                xs[i][0] = make_float2(z.x + z.y, 0.0f);
+                xs[i][1] = make_float2(z.x - z.y, 0.0f);
+            } else {
+                kiss_post(lds.ld(k), lds.ld(NC - k), pw[i], xs[i][0], xs[i][1]);
+            }
+        }
         const long g0 = (long)blockIdx.x * SPB;
         float *re0 = out_re + g0 * NB, *im0 = out_im + g0 * NB;
         if (out_dist == NB && g0 + SPB <= count &&
             ((((uintptr_t)re0) | ((uintptr_t)im0)) & 15) == 0) {
-            float2 xs[NP][2];
-#pragma unroll
-            for (int i = 0; i < NP; ++i) {
-                const int k = lt + i * T;
-                if (k > NC / 2) break;
-                if (k == 0) {
-                    float2 z = lds.ld(0);
-                    xs[i][0] = make_float2(z.x + z.y, 0.0f);
-                    xs[i][1] = make_float2(z.x - z.y, 0.0f);
-                } else {
-                    kiss_post(lds.ld(k), lds.ld(NC - k), pw[i], xs[i][0], xs[i][1]);
-                }
-            }
             __syncthreads();                      /* every signal is out of LDS */
             float *sre = (float *)faf_smem, *sim = sre + RUN;
 #pragma unroll
@@ -295,6 +298,16 @@ void kc_r2c(fafd_fft P, const float2 *post, const float *x, long in_dist, float 
             }
             return;
         }
+        if (!active) return;
+        float *ore = out_re + sig * out_dist, *oim = out_im + sig * out_dist;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const int k = lt + i * T;
+            if (k > NC / 2) break;
+            ore[k] = xs[i][0].x * scale; oim[k] = xs[i][0].y * scale;
+            ore[NC - k] = xs[i][1].x * scale; oim[NC - k] = xs[i][1].y * scale;
+        }
+        return;
     }
     if (!active) return;
     float *ore = out_re + sig * out_dist;
@@ -576,23 +589,24 @@ void kc_stft(fafd_fft P, const float2 *post, const float *win, const float *x, l
      * instructions, every one a full contiguous line set. */
     constexpr int NB = NC + 1, RUN = SPB * NB;
     if constexpr (FULL && RUN % 4 == 0 && RUN * 2 * 4 <= SPB * (NC + NC / 16 + 1) * 8) {
+        /* bins formed once, then stored either way (as in kc_r2c) */
+        float2 xs[NP][2];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const int k = lt + i * T;
+            if (k > NC / 2) break;
+            if (k == 0) {
+                float2 z = lds.ld(0);
+                xs[i][0] = make_float2(z.x + z.y, 0.0f);
+                xs[i][1] = make_float2(z.x - z.y, 0.0f);
+            } else {
+                kiss_post(lds.ld(k), lds.ld(NC - k), pw[i], xs[i][0], xs[i][1]);
+            }
+        }
         const long g0 = (long)blockIdx.x * SPB;
         float *re0 = out_re + g0 * NB, *im0 = out_im + g0 * NB;
         if (frame_dist == NB && rec_out_dist == frames * NB && g0 + SPB <= count &&
             ((((uintptr_t)re0) | ((uintptr_t)im0)) & 15) == 0) {
-            float2 xs[NP][2];
-#pragma unroll
-            for (int i = 0; i < NP; ++i) {
-                const int k = lt + i * T;
-                if (k > NC / 2) break;
-                if (k == 0) {
-                    float2 z = lds.ld(0);
-                    xs[i][0] = make_float2(z.x + z.y, 0.0f);
-                    xs[i][1] = make_float2(z.x - z.y, 0.0f);
-                } else {
-                    kiss_post(lds.ld(k), lds.ld(NC - k), pw[i], xs[i][0], xs[i][1]);
-                }
-            }
             __syncthreads();                      /* every frame is out of LDS */
             float *sre = (float *)faf_smem, *sim = sre + RUN;
 #pragma unroll
@@ -611,6 +625,17 @@ void kc_stft(fafd_fft P, const float2 *post, const float *win, const float *x, l
             }
             return;
         }
+        if (!active) return;
+        float *ore = out_re + rec * rec_out_dist + f * frame_dist;
+        float *oim = out_im + rec * rec_out_dist + f * frame_dist;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const int k = lt + i * T;
+            if (k > NC / 2) break;
+            ore[k] = xs[i][0].x; oim[k] = xs[i][0].y;
+            ore[NC - k] = xs[i][1].x; oim[NC - k] = xs[i][1].y;
+        }
+        return;
     }
     if (!active) return;
     float *ore = out_re + rec * rec_out_dist + f * frame_dist;
