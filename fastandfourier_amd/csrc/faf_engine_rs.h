@@ -522,6 +522,37 @@ __global__ __launch_bounds__(256) void kr8_r2c(const float2 *tw, const float2 *p
     }
 }
 
+/* C2C 8192 in one pass, one workgroup per signal, split or interleaved in
+ * and out (interleaved dists in complex elements, split in floats) */
+template <bool INV, bool INTER_IN, bool INTER_OUT>
+__global__ __launch_bounds__(256) void kr8_c2c(const float2 *tw, const float *in_re,
+                                               const float *in_im, long in_dist,
+                                               float *out_re, float *out_im, long out_dist,
+                                               float scale) {
+    using namespace rs8;
+    float *lds = (float *)faf_smem;
+    const int tid = (int)threadIdx.x;
+    const long s = blockIdx.x;
+    float2 v[32], b1[4][3];
+    const float *ire = in_re + s * in_dist, *iim = in_im + s * in_dist;
+    const float2 *ii = (const float2 *)in_re + s * in_dist;
+#pragma unroll
+    for (int m = 0; m < 32; ++m) {
+        const int k = tid + T * m;
+        v[m] = INTER_IN ? ii[k] : make_float2(ire[k], iim[k]);
+    }
+    tw1_bases(tw, tid, b1);
+    fft8192<INV>(v, lds, tw, b1, tid);
+    float *ore = out_re + s * out_dist, *oim = out_im + s * out_dist;
+    float2 *oi = (float2 *)out_re + s * out_dist;
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+        const int k = tid + T * j;
+        if (INTER_OUT) oi[k] = cscale(v[j], scale);
+        else { ore[k] = v[j].x * scale; oim[k] = v[j].y * scale; }
+    }
+}
+
 /* irfft N = 16384 in one pass: N/2+1 bins in, y real (8-byte aligned
  * rows); the Kiss pre-pass is kr_c2r's with 256 threads */
 template <bool INTER_IN>

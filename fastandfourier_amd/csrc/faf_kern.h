@@ -48,6 +48,10 @@ int fafk_rs2_pipe(const void *twf, const void *twi, const void *post, const void
 int fafk_rs2_c2r(int inter_in, const void *tw, const void *pre, const float *in_re,
                  const float *in_im, long in_dist, float *y, long out_dist, long count,
                  float scale, void *stream);
+/* C2C 8192 in one pass: tw = W_8192 table of the direction */
+int fafk_rs8_c2c(int inv, int inter, const void *tw, const float *in_re, const float *in_im,
+                 long in_dist, float *out_re, float *out_im, long out_dist, long count,
+                 float scale, void *stream);
 int fafk_rs8_r2c(int inter_out, const void *tw, const void *post, const float *x, long in_dist,
                  float *out_re, float *out_im, long out_dist, long count, float scale,
                  void *stream);

@@ -98,6 +98,27 @@ extern "C" int fafk_rs2_pipe(const void *twf, const void *twi, const void *post,
     return e == hipSuccess ? 0 : rs_fail("kr2_pipe", e);
 }
 
+/* C2C 8192 in one pass (rs8): one 256-thread workgroup per signal */
+extern "C" int fafk_rs8_c2c(int inv, int inter, const void *tw, const float *in_re,
+                            const float *in_im, long in_dist, float *out_re, float *out_im,
+                            long out_dist, long count, float scale, void *stream) {
+    if (count <= 0) return 0;
+    if (count > 0x7fffffffL) return rs_fail("kr8_c2c", hipErrorInvalidValue);
+    const dim3 g((unsigned)count), b(rs8::T);
+    const float2 *t = (const float2 *)tw;
+#define RS8_C2C(I, L)                                                                       \
+    hipLaunchKernelGGL((kr8_c2c<I, L, L>), g, b, rs8::LDS_BYTES, S(stream), t, in_re, in_im, \
+                       in_dist, out_re, out_im, out_dist, scale)
+    if (inv) {
+        if (inter) RS8_C2C(true, true); else RS8_C2C(true, false);
+    } else {
+        if (inter) RS8_C2C(false, true); else RS8_C2C(false, false);
+    }
+#undef RS8_C2C
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : rs_fail("kr8_c2c", e);
+}
+
 /* float2 row access needs 8-byte aligned rows */
 extern "C" int fafk_rs_pair_ok(const void *p, long dist) {
     return ((uintptr_t)p & 7u) == 0 && (dist & 1) == 0;

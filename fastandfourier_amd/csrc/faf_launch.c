@@ -68,6 +68,9 @@ int fafi_run_c2c(fafh_plan *p, const float *ir, const float *ii, long idist, flo
     const int inter = p->interleaved;
     switch (p->kind) {
     case PK_C2C:
+        if (p->ct && p->n == 8192)   /* one pass, every count and layout */
+            return krerr(fafk_rs8_c2c(p->inverse, inter, p->a.tw, ir, ii, idist, orr, oi, odist,
+                                      count, p->scale, stream));
         if (p->ct)
             return kcerr(fafk_ct_c2c(&p->a, p->inverse, inter, ir, ii, idist, orr, oi, odist,
                                      count, p->scale, stream));
