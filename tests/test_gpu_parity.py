@@ -106,6 +106,30 @@ def test_c2c_norms(norm):
         G.destroy(ti)
 
 
+@pytest.mark.parametrize("n", [2048, 8192])
+@pytest.mark.parametrize("inter", [False, True])
+def test_c2c_batches_2048_8192(n, inter):
+    """Batched C2C at 2048 (staged passes) and 8192 (the single-pass
+    register-resident core), split and interleaved, both directions, against
+    FP64; a signal alone gives the same bits as inside the batch."""
+    say(f"case: single-pass c2c n={n} inter={inter}")
+    rng = np.random.default_rng(n + (7 if inter else 0))
+    x = (rng.uniform(-1, 1, (9, n)) + 1j * rng.uniform(-1, 1, (9, n))).astype(np.complex64)
+    kw = {"layout": "interleaved"} if inter else {}
+    t = G.create("fft", n, **kw)
+    ti = G.create("fft", n, dir="inverse", **kw)
+    try:
+        X = G.c2c(t, x, inter=inter)
+        assert rel_l2(X, np.fft.fft(x.astype(np.complex128), axis=1)) < tol(n)
+        y = G.c2c(ti, X, inter=inter)
+        assert rel_l2(y, x) < 2 * tol(n)
+        for i in (0, 8):
+            assert np.array_equal(G.c2c(t, x[i:i + 1], inter=inter)[0], X[i])
+    finally:
+        G.destroy(t)
+        G.destroy(ti)
+
+
 @pytest.mark.parametrize("n", [64, 4096, 32768, 65536])
 def test_c2c_interleaved(n):
     say(f"case: c2c interleaved n={n}")
