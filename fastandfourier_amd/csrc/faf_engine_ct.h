@@ -574,7 +574,17 @@ void kc_stft(fafd_fft P, const float2 *post, const float *win, const float *x, l
         const int k = lt + i * T;
         if (active && k > 0 && k <= NC / 2) pw[i] = post[k];
     }
-    if constexpr (FULL) {
+    if constexpr (FULL && NC <= 512) {
+        /* the window (2 NC floats, at most 4 KB) is copied once into LDS
+         * behind the frame planes; every frame of the workgroup reads it from
+         * there instead of issuing its own window loads (STFT 64 x 2^22,
+         * window 1024: 1.734 -> 1.442 ms).  Four workgroups still fit a CU. */
+        float *wl = (float *)(faf_smem + SPB * P.plane);
+        for (int i = tid; i < 2 * NC; i += SPB * T) wl[i] = win[i];
+        __syncthreads();
+        GFrameFull src{x + rec * rec_dist + f * hop, wl};
+        ch.run(src, lds, lds, lt, active);
+    } else if constexpr (FULL) {
         GFrameFull src{x + rec * rec_dist + f * hop, win};
         ch.run(src, lds, lds, lt, active);
     } else {

@@ -238,7 +238,7 @@ extern "C" int fafk_ct_stft(const fafd_fft *P, const void *post, const float *wi
     const bool full = rec_len >= 2L * P->n && P->n <= 8192;
 #define STFT(N)                                                                          \
     do {                                                                                 \
-        if (full) hipLaunchKernelGGL((kc_stft<true, N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, po, win, x, rec_len, rec_dist, hop, frames, out_re, out_im, frame_dist, rec_out_dist, count); \
+        if (full) hipLaunchKernelGGL((kc_stft<true, N>), GRID(N), BLOCK(N), LDS(N, P->plane) + (N <= 512 ? 8 * N : 0), S(stream), *P, po, win, x, rec_len, rec_dist, hop, frames, out_re, out_im, frame_dist, rec_out_dist, count); \
         else hipLaunchKernelGGL((kc_stft<false, N>), GRID(N), BLOCK(N), LDS(N, P->plane), S(stream), *P, po, win, x, rec_len, rec_dist, hop, frames, out_re, out_im, frame_dist, rec_out_dist, count); \
     } while (0)
     CT_SWITCH(P->n, STFT)
