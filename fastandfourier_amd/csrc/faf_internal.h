@@ -71,6 +71,9 @@ typedef struct fafh_plan {
     int interleaved;          /* buffers are interleaved complex */
     void *ws;                 /* device workspace */
     size_t ws_bytes;
+    int ws_captured;          /* a graph holds ws: growth retires it, not frees it */
+    void **ws_retired;        /* earlier workspaces kept for captured graphs */
+    int n_retired;
     void *stg;                /* device staging for host buffers */
     size_t stg_bytes;
     /* pipeline */

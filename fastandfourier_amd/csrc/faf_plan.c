@@ -367,6 +367,8 @@ void fafi_plan_free(fafh_plan *p) {
     for (int i = 0; i < 3; ++i) fafk_free(p->rader_mem[i]);
     for (int i = 0; i < 2; ++i) fafk_free(p->blue_mem[i]);
     fafk_free(p->ws);
+    for (int i = 0; i < p->n_retired; ++i) fafk_free(p->ws_retired[i]);
+    free(p->ws_retired);
     fafk_free(p->stg);
     fafk_free(p->h_dev);
     fafk_free(p->win_dev);
@@ -379,14 +381,28 @@ void *fafi_workspace(fafh_plan *p, size_t bytes) {
         fafi_err("lent workspace holds %zu bytes, the run needs %zu", p->ws_ext_bytes, bytes);
         return NULL;
     }
-    if (p->ws_bytes >= bytes) return p->ws;
+    if (p->ws_bytes >= bytes) {
+        /* a graph captured now keeps this buffer's address */
+        if (fafi_capturing()) p->ws_captured = 1;
+        return p->ws;
+    }
     if (fafi_capturing()) {
         fafi_err("the workspace must grow to %zu bytes, which a graph capture cannot do; "
                  "execute once before capturing", bytes);
         return NULL;
     }
     fafk_sync(fafi_stream());
-    fafk_free(p->ws);
+    if (p->ws_captured) {
+        /* graphs may still replay on the old buffer: keep it until the plan
+         * is destroyed instead of freeing it under them */
+        void **rt = realloc(p->ws_retired, (size_t)(p->n_retired + 1) * sizeof(void *));
+        if (!rt) { fafi_err("workspace: out of host memory"); return NULL; }
+        p->ws_retired = rt;
+        p->ws_retired[p->n_retired++] = p->ws;
+        p->ws_captured = 0;
+    } else {
+        fafk_free(p->ws);
+    }
     p->ws = fafk_malloc(bytes);
     p->ws_bytes = p->ws ? bytes : 0;
     if (!p->ws) fafi_err("%s", fafk_last_error());

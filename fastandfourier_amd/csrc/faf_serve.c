@@ -122,8 +122,14 @@ struct faf_server {
     int hash[2 * SV_MAX_ENTRIES];   /* open addressing: entry index + 1, 0 empty */
     fafd_splan *splans_dev;
     int nsplan;
-    /* request records, in pages indexed by ticket */
+    /* request records, in pages of SV_PAGE tickets; page number pg lives in
+     * slot pg % SV_MAX_PAGES, and page_no[] says which page a slot holds
+     * (-1 when empty).  A slot is recycled once every request of its page
+     * has finished, its result is freed and the stats window has moved past
+     * it, so the server runs indefinitely with a bounded record store.
+     * next_ticket only advances under in_mu, after the record is filled. */
     srec *pages[SV_MAX_PAGES];
+    int64_t page_no[SV_MAX_PAGES];
     _Atomic int64_t next_ticket;
     /* submissions and releases, handed to the worker */
     pthread_mutex_t in_mu;
@@ -209,7 +215,13 @@ static int64_t ring_pop(ring *r) {
 }
 
 static srec *rec(faf_server *s, int64_t t) {
-    return &s->pages[t >> SV_PAGE_BITS][t & (SV_PAGE - 1)];
+    return &s->pages[(t >> SV_PAGE_BITS) % SV_MAX_PAGES][t & (SV_PAGE - 1)];
+}
+
+/* 1 if ticket t's record is still stored (its page was not recycled) */
+static int rec_live(faf_server *s, int64_t t) {
+    const int64_t pg = t >> SV_PAGE_BITS;
+    return s->pages[pg % SV_MAX_PAGES] && s->page_no[pg % SV_MAX_PAGES] == pg;
 }
 
 /* ------------------------------------------------------------------ */
@@ -634,7 +646,9 @@ static int take_inbox(faf_server *s) {
         did = 1;
     }
     while (s->rel.n) {
-        srec *r = rec(s, ring_pop(&s->rel));
+        const int64_t t = ring_pop(&s->rel);
+        if (!rec_live(s, t)) continue;     /* recycled after an earlier release */
+        srec *r = rec(s, t);
         if (!r->freed) {
             arena_free(s, r->blk, r->blk_bytes);
             r->freed = 1;
@@ -1104,6 +1118,7 @@ faf_server *faf_server_create(const faf_server_config *cfg) {
     pthread_mutex_init(&s->done_mu, NULL);
     pthread_cond_init(&s->in_cv, NULL);
     pthread_cond_init(&s->done_cv, NULL);
+    for (int i = 0; i < SV_MAX_PAGES; ++i) s->page_no[i] = -1;
     s->arena_bytes = c.hbm_budget & ~(size_t)(SV_ALIGN - 1);
     s->arena = fafk_malloc(s->arena_bytes);
     s->splans_dev = fafk_malloc(SV_MAX_ENTRIES * sizeof(fafd_splan));
@@ -1166,21 +1181,55 @@ size_t faf_server_result_bytes(faf_server *s, faf_req_kind kind, size_t n) {
     return e ? result_bytes(s, e, n) : 0;
 }
 
+/* Frees the page in slot `sl` if every request in it has finished with its
+ * result freed and the stats window starts past it.  Called with in_mu held
+ * (no submit can touch the page); takes done_mu for the records. */
+static int page_reclaim(faf_server *s, int sl) {
+    const int64_t pg = s->page_no[sl];
+    int ok = 1;
+    pthread_mutex_lock(&s->done_mu);
+    if ((pg + 1) * SV_PAGE > s->stats_first) ok = 0;
+    for (int64_t i = 0; ok && i < SV_PAGE; ++i) {
+        srec *r = &s->pages[sl][i];
+        int st = atomic_load(&r->state);
+        if ((st != ST_DONE && st != ST_FAILED) || !r->freed) ok = 0;
+    }
+    if (ok) {
+        free(s->pages[sl]);
+        s->pages[sl] = NULL;
+        s->page_no[sl] = -1;
+    }
+    pthread_mutex_unlock(&s->done_mu);
+    return ok;
+}
+
 static int64_t submit_at(faf_server *s, int kind, size_t n, const float *in, uint32_t flags,
                          uint64_t t_arr) {
     if (!s || !in) { fafi_err("serve: null argument"); return -1; }
     if (kind == FAF_REQ_STFT && n < 1) { fafi_err("serve: empty STFT record"); return -1; }
     sentry *e = entry_get(s, kind, n);
     if (!e) return -1;
-    int64_t t = atomic_fetch_add(&s->next_ticket, 1);
-    if ((t >> SV_PAGE_BITS) >= SV_MAX_PAGES) { fafi_err("serve: ticket space exhausted"); return -1; }
+    /* the ticket is issued only once its record exists: a submission that
+     * fails here leaves no ticket behind for drain or stats to wait on */
     pthread_mutex_lock(&s->in_mu);
-    srec **pg = &s->pages[t >> SV_PAGE_BITS];
-    if (!*pg) *pg = calloc((size_t)SV_PAGE, sizeof(srec));
-    if (!*pg) {
-        pthread_mutex_unlock(&s->in_mu);
-        fafi_err("serve: alloc failed");
-        return -1;
+    const int64_t t = atomic_load(&s->next_ticket);
+    const int64_t pg = t >> SV_PAGE_BITS;
+    const int sl = (int)(pg % SV_MAX_PAGES);
+    if (s->page_no[sl] != pg) {
+        if (s->pages[sl] && !page_reclaim(s, sl)) {
+            pthread_mutex_unlock(&s->in_mu);
+            fafi_err("serve: %lld requests are unreleased or inside the stats window; "
+                     "release results (or submit with FAF_REQ_DISCARD) and reset the stats",
+                     (long long)(SV_MAX_PAGES * SV_PAGE));
+            return -1;
+        }
+        s->pages[sl] = calloc((size_t)SV_PAGE, sizeof(srec));
+        if (!s->pages[sl]) {
+            pthread_mutex_unlock(&s->in_mu);
+            fafi_err("serve: alloc failed");
+            return -1;
+        }
+        s->page_no[sl] = pg;
     }
     srec *r = rec(s, t);
     memset(r, 0, sizeof *r);
@@ -1194,20 +1243,15 @@ static int64_t submit_at(faf_server *s, int kind, size_t n, const float *in, uin
     r->io_bytes = input_bytes(kind, n) + (double)r->out_bytes;
     r->t_arr = t_arr;
     atomic_store(&r->state, ST_QUEUED);
-    int rc = ring_push(&s->inbox, t);
-    if (s->sleeping) pthread_cond_signal(&s->in_cv);
-    pthread_mutex_unlock(&s->in_mu);
-    if (rc) {
-        /* the record exists but was never queued: mark it failed */
-        pthread_mutex_lock(&s->done_mu);
-        atomic_store(&r->state, ST_FAILED);
-        r->freed = 1;
-        s->finished++;
-        pthread_cond_broadcast(&s->done_cv);
-        pthread_mutex_unlock(&s->done_mu);
+    if (ring_push(&s->inbox, t)) {
+        /* not queued and no ticket issued: the record slot is reused */
+        pthread_mutex_unlock(&s->in_mu);
         fafi_err("serve: alloc failed");
         return -1;
     }
+    atomic_store(&s->next_ticket, t + 1);
+    if (s->sleeping) pthread_cond_signal(&s->in_cv);
+    pthread_mutex_unlock(&s->in_mu);
     return t;
 }
 
@@ -1219,6 +1263,10 @@ int64_t faf_server_submit(faf_server *s, faf_req_kind kind, size_t n, const floa
 static srec *valid(faf_server *s, int64_t t) {
     if (!s || t < 0 || t >= atomic_load(&s->next_ticket)) {
         fafi_err("serve: invalid ticket %lld", (long long)t);
+        return NULL;
+    }
+    if (!rec_live(s, t)) {
+        fafi_err("serve: ticket %lld has expired (released and recycled)", (long long)t);
         return NULL;
     }
     return rec(s, t);
@@ -1289,6 +1337,7 @@ int faf_server_stats_get(faf_server *s, faf_server_stats *st) {
     double bytes = 0.0, sum = 0.0;
     pthread_mutex_lock(&s->done_mu);
     for (int64_t t = lo; t < hi; ++t) {
+        if (!rec_live(s, t)) continue;
         srec *r = rec(s, t);
         int state = atomic_load(&r->state);
         st->submitted++;
@@ -1337,6 +1386,11 @@ void faf_server_stats_reset(faf_server *s) {
     s->stats_first = atomic_load(&s->next_ticket);
     atomic_store(&s->peak, atomic_load(&s->in_use));
     pthread_mutex_unlock(&s->done_mu);
+    /* pages now behind the stats window whose results are all freed go */
+    pthread_mutex_lock(&s->in_mu);
+    for (int i = 0; i < SV_MAX_PAGES; ++i)
+        if (s->pages[i]) (void)page_reclaim(s, i);
+    pthread_mutex_unlock(&s->in_mu);
     atomic_store(&s->launches, 0);
     atomic_store(&s->launched_reqs, 0);
     atomic_store(&s->long_reqs, 0);
@@ -1364,7 +1418,10 @@ int faf_server_replay(faf_server *s, const int *kinds, const size_t *ns,
     }
     faf_server_drain(s);
     for (size_t i = 0; i < count; ++i) {
-        if (tk[i] < 0) { if (latency_ns) latency_ns[i] = 0; continue; }
+        if (tk[i] < 0 || !rec_live(s, tk[i])) {
+            if (latency_ns) latency_ns[i] = 0;
+            continue;
+        }
         srec *r = rec(s, tk[i]);
         if (atomic_load(&r->state) != ST_DONE) rc = -1;
         if (latency_ns) latency_ns[i] = r->t_done > r->t_arr ? r->t_done - r->t_arr : 0;

@@ -175,6 +175,47 @@ def test_graph_c2c_four_step_replays():
         G.destroy(t)
 
 
+def test_graph_survives_workspace_growth():
+    """A graph keeps the workspace address it captured.  An eager execute at
+    a larger batch afterwards grows the plan's workspace: the captured buffer
+    is retired (kept until the plan is destroyed), not freed, so replays
+    still run on valid memory and give the eager result."""
+    say("case: graph c2c 65536 after workspace growth")
+    lib = G.L()
+    n, b = 65536, 2
+    rng = np.random.default_rng(n + 13)
+    t = G.create("fft", n)
+    xr, xi, yr, yi, er, ei = (G.empty((b, n)) for _ in range(6))
+    bx = fa.buffer_split(xr.data_ptr(), xi.data_ptr(), n)
+    by = fa.buffer_split(yr.data_ptr(), yi.data_ptr(), n)
+    be = fa.buffer_split(er.data_ptr(), ei.data_ptr(), n)
+    g = None
+    try:
+        fa.check(lib.faf_execute_batch(t, C.byref(by), C.byref(bx), b, 0, 0), "eager")
+        g = capture(lambda: [lib.faf_execute_batch(t, C.byref(by), C.byref(bx), b, 0, 0)])
+        for big in (8, 24):                   # each grows the workspace past the captured one
+            ur, ui, vr, vi = (G.empty((big, n)) for _ in range(4))
+            u = cplx(rng, big, n)
+            put(ur, u.real)
+            put(ui, u.imag)
+            bu = fa.buffer_split(ur.data_ptr(), ui.data_ptr(), n)
+            bv = fa.buffer_split(vr.data_ptr(), vi.data_ptr(), n)
+            fa.check(lib.faf_execute_batch(t, C.byref(bv), C.byref(bu), big, 0, 0), "grow")
+            v = G.host(vr) + 1j * G.host(vi)
+            assert rel_l2(v[big - 1], O.fft64(u[big - 1].astype(np.complex128))) < tol(n)
+            x = cplx(rng, b, n)
+            put(xr, x.real)
+            put(xi, x.imag)
+            fa.check(lib.faf_execute_batch(t, C.byref(be), C.byref(bx), b, 0, 0), "eager")
+            fa.check(lib.faf_graph_launch(g), "graph launch")
+            y = G.host(yr) + 1j * G.host(yi)
+            assert np.array_equal(y, G.host(er) + 1j * G.host(ei)), f"after growth to {big}"
+            assert rel_l2(y, O.fft64(x.astype(np.complex128))) < tol(n)
+    finally:
+        lib.faf_graph_destroy(g)
+        G.destroy(t)
+
+
 def test_graph_ragged_replays():
     """The ragged launch resets its work counter with a fill kernel, so replays
     start from an empty queue every time."""

@@ -278,6 +278,36 @@ def test_serve_replay_poisson_stream():
         lib.faf_server_destroy(s)
 
 
+def test_serve_failed_submit_issues_no_ticket():
+    """A submission that fails returns -1 and leaves no ticket behind, so
+    drain, stats and later submissions go on as if it never happened."""
+    say("case: serve failed submissions")
+    lib = G.L()
+    rng = np.random.default_rng(21)
+    s = server(hbm_budget=1 << 28)
+    try:
+        x, dx = make_input(rng, "rfft", 1024)
+        assert lib.faf_server_submit(s, 0, 1023, dx.data_ptr(), 0) == -1      # odd rfft
+        assert lib.faf_server_submit(s, 0, 1024, None, 0) == -1               # no input
+        assert lib.faf_server_submit(s, 9, 1024, dx.data_ptr(), 0) == -1      # no such kind
+        fa.check(lib.faf_server_drain(s), "drain after failures")
+        t0 = lib.faf_server_submit(s, 0, 1024, dx.data_ptr(), 0)
+        assert t0 == 0, "failed submissions must not consume tickets"
+        y = fetch(s, t0)
+        assert rel_l2(y, O.fft64(x.astype(np.complex128)).ravel()[:513]) < tol(1024)
+        fa.check(lib.faf_server_release(s, t0), "release")
+        fa.check(lib.faf_server_drain(s), "drain")
+        st = fa.FafServerStats()
+        lib.faf_server_stats_get(s, C.byref(st))
+        assert st.submitted == 1 and st.completed == 1 and st.failed == 0
+        lib.faf_server_stats_reset(s)
+        lib.faf_server_stats_get(s, C.byref(st))
+        assert st.submitted == 0
+        assert lib.faf_server_poll(s, 5) == -1                                # never issued
+    finally:
+        lib.faf_server_destroy(s)
+
+
 @pytest.mark.parametrize("n", [16384, 32768, 65536])
 def test_cwt_long_segments_match_fp64(n):
     """CWT segments longer than one workgroup: rfft, spread, batched inverse."""
