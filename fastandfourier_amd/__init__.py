@@ -168,6 +168,7 @@ _SIGS = {
     "faf_graph_launch": (c_int, [c_void_p]), "faf_graph_nodes": (c_int, [c_void_p]),
     "faf_graph_destroy": (None, [c_void_p]),
     "faf_execute_batch": (c_int, [PT, PB, PB, c_size_t, c_size_t, c_size_t]),
+    "faf_execute_batch_noalloc": (c_int, [PT, PB, PB, c_size_t, c_size_t, c_size_t]),
     "faf_ragged_create": (c_void_p, [POINTER(c_size_t), c_size_t, c_uint32]),
     "faf_ragged_total": (c_size_t, [c_void_p]),
     "faf_ragged_execute": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

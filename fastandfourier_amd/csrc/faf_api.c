@@ -101,6 +101,10 @@ struct faf_graph {
 
 int fafi_capturing(void) { return t_capture; }
 
+/* 1 while faf_execute_batch_noalloc runs on this thread */
+static _Thread_local int t_noalloc;
+int fafi_noalloc(void) { return t_noalloc; }
+
 /* create-time entry: plans upload their tables synchronously, which a
  * capture cannot hold */
 int fafi_create_ready(void) {
@@ -566,6 +570,11 @@ static int planes_stage_in(fafh_plan *p, planes_t *ps, void *stream) {
         fafi_err("host buffers cannot be captured into a graph; pass device pointers");
         return -1;
     }
+    if (t_noalloc) {
+        fafi_err("faf_execute_batch_noalloc: every buffer must be device memory "
+                 "(host buffers need staging memory)");
+        return -1;
+    }
     char *base = fafi_staging(p, need);
     if (!base) return -1;
     size_t off = 0;
@@ -609,7 +618,7 @@ static int exec_planes(const faf_transform *t, planes_t *ps, long count, long id
         return -1;
     }
     void *ev0 = NULL, *ev1 = NULL;
-    if (t_timing && !t_capture) {
+    if (t_timing && !t_capture && !t_noalloc) {
         ev0 = fafk_event_create();
         ev1 = fafk_event_create();
         if (ev0) fafk_event_record(ev0, stream);
@@ -823,6 +832,36 @@ int faf_execute(const faf_transform *t, faf_buffer *out, const faf_buffer *in) {
 int faf_execute_batch(const faf_transform *t, faf_buffer *out, const faf_buffer *in,
                       size_t count, size_t in_dist, size_t out_dist) {
     return execute_n(t, out, in, count, in_dist, out_dist);
+}
+
+_Static_assert(FAFK_SCRATCH_BYTES == FAF_NOALLOC_SCRATCH_BYTES,
+               "the kernel layer's scratch is the size the header states");
+
+int faf_execute_batch_noalloc(const faf_transform *t, faf_buffer *out, const faf_buffer *in,
+                              size_t count, size_t in_dist, size_t out_dist) {
+    if (!t || !out || !in) { fafi_err("faf_execute_batch_noalloc: null argument"); return -1; }
+    fafh_plan *p = t->amd;
+    if (!p) { fafi_err("transform has no device plan"); return -1; }
+    if (fafi_runtime_ready()) return -1;
+    size_t cap = 0;
+    void *scr = fafk_scratch(&cap);   /* this device's copy of the static scratch */
+    if (!scr) { fafi_err("faf_execute_batch_noalloc: %s", fafk_last_error()); return -1; }
+    /* the plan's own workspace is replaced by the scratch for this call:
+     * four-step runs cut their batch into chunks that fit it, and a run
+     * that cannot fit (one chunk larger than the scratch) fails in
+     * fafi_workspace with the sizes */
+    void *save = p->ws_ext;
+    size_t save_b = p->ws_ext_bytes;
+    p->ws_ext = scr;
+    p->ws_ext_bytes = cap;
+    t_noalloc = 1;
+    fafk_set_noalloc(1);
+    int rc = execute_n(t, out, in, count, in_dist, out_dist);
+    fafk_set_noalloc(0);
+    t_noalloc = 0;
+    p->ws_ext = save;
+    p->ws_ext_bytes = save_b;
+    return rc;
 }
 
 int faf_execute_f32(const faf_transform *t, float *out, const float *in) {

@@ -241,6 +241,10 @@ int fafi_pipe_refresh(fafh_plan *p, void *stream) {
             fafi_err("a graph capture needs the bound spectrum in device memory");
             return -1;
         }
+        if (fafi_noalloc()) {
+            fafi_err("faf_execute_batch_noalloc: the bound spectrum must be device memory");
+            return -1;
+        }
         if (p->h_dev_cap < cap) {
             fafk_free(p->h_dev);
             p->h_dev = fafk_malloc(cap);

@@ -25,6 +25,7 @@ int fafi_is_codelet_size(size_t n);
 int fafi_runtime_ready(void);         /* 0 ok, -1 with error set */
 int fafi_create_ready(void);          /* as above, and no graph capture open */
 int fafi_capturing(void);             /* 1 while this thread captures a graph */
+int fafi_noalloc(void);               /* 1 inside faf_execute_batch_noalloc */
 void *fafi_stream(void);              /* current thread's stream (NULL = default) */
 int fafi_sync_mode(void);
 int fafi_ct_enabled(void);            /* plans may pick compile-time kernels */

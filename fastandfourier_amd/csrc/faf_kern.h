@@ -15,6 +15,13 @@ extern "C" {
 
 const char *fafk_last_error(void);
 int fafk_init(void);
+/* the static scratch of the allocation-free entry point: its device address
+ * on the current device (resolved once per device) and its size */
+#define FAFK_SCRATCH_BYTES ((size_t)32 << 20)   /* = FAF_NOALLOC_SCRATCH_BYTES */
+void *fafk_scratch(size_t *bytes);
+/* thread-local: while on, fafk_malloc and the plan-record cache refuse to
+ * allocate (they fail with a message instead) */
+void fafk_set_noalloc(int on);
 void *fafk_malloc(size_t bytes);
 void fafk_free(void *p);
 int fafk_h2d(void *dst, const void *src, size_t bytes, void *stream);

@@ -53,6 +53,55 @@ template <int M> static void init_attrs() {
     lds_attr(k_serve<M>);
 }
 
+/* ---------------- static scratch of the allocation-free entry point ----
+ * faf_execute_batch_noalloc lends this array to a plan as its workspace
+ * (four-step intermediates and the like) instead of allocating one.  Every
+ * GPU that loads the library holds its own copy, at its own address, until
+ * the process ends.  Host code never passes the array's name to a kernel
+ * (on the host it names a stand-in, not device memory): its device address
+ * is looked up once per device with hipGetSymbolAddress and kept in
+ * g_scratch_ptr[device]. */
+static __device__ __align__(256) unsigned char scratch[FAFK_SCRATCH_BYTES];
+
+#define FAFK_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[FAFK_MAX_DEVICES];
+
+/* resolves the current device's scratch address (once per device) */
+static int scratch_resolve(void **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail("hipGetDevice", e);
+    if (dev < 0 || dev >= FAFK_MAX_DEVICES) {
+        snprintf(g_err, sizeof g_err, "device %d beyond the scratch table", dev);
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return fail("hipGetSymbolAddress (scratch)", e);
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+extern "C" void *fafk_scratch(size_t *bytes) {
+    void *p = nullptr;
+    if (scratch_resolve(&p)) return nullptr;
+    if (bytes) *bytes = FAFK_SCRATCH_BYTES;
+    return p;
+}
+
+/* while set on a thread, device-memory allocations of this layer fail
+ * instead of happening (the plan-record cache included) */
+static thread_local int t_noalloc;
+extern "C" void fafk_set_noalloc(int on) { t_noalloc = on; }
+
 extern "C" int fafk_init(void) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
@@ -60,6 +109,10 @@ extern "C" int fafk_init(void) {
         snprintf(g_err, sizeof g_err, "no HIP device available (%s)",
                  e == hipSuccess ? "count 0" : hipGetErrorString(e));
         return -1;
+    }
+    {   /* the scratch address of the device current at set-up */
+        void *p = nullptr;
+        if (scratch_resolve(&p)) return -1;
     }
     init_attrs<512>();
     init_attrs<1024>();
@@ -77,6 +130,11 @@ extern "C" int fafk_init(void) {
 
 extern "C" void *fafk_malloc(size_t bytes) {
     void *p = nullptr;
+    if (t_noalloc) {
+        snprintf(g_err, sizeof g_err, "allocation of %zu bytes refused: the call may not "
+                 "allocate device memory", bytes);
+        return nullptr;
+    }
     hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
     if (e != hipSuccess) { fail("hipMalloc", e); return nullptr; }
     return p;
@@ -254,6 +312,12 @@ static const void *dev_rec(const void *h, size_t bytes) {
     auto it = g_recs.find(key);
     if (it != g_recs.end()) return it->second;
     const size_t need = (bytes + 255) & ~(size_t)255;
+    if (t_noalloc) {
+        /* a record not yet uploaded would need a copy (and maybe a slab) */
+        snprintf(g_err, sizeof g_err, "device plan record not resident: run the plan once "
+                 "through faf_execute_batch before the allocation-free call");
+        return nullptr;
+    }
     if (g_slab_left < need) {
         const size_t sz = need > 65536 ? need : 65536;
         void *p = nullptr;

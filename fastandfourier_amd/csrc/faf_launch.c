@@ -35,8 +35,15 @@ static int krerr(int rc) {
 
 static int rs_grid(void) { return (int)fafi_cus(); }
 
-static long chunk_signals(size_t n_complex, long count) {
+/* signals per four-step chunk; per_sig = workspace bytes one signal takes.
+ * A lent workspace (the serving arena, the static scratch of the
+ * allocation-free call) caps the chunk at what it holds. */
+static long chunk_signals(const fafh_plan *p, size_t n_complex, size_t per_sig, long count) {
     long c = (long)(CHUNK_BYTES / (n_complex * 2 * sizeof(float)));
+    if (p->ws_ext) {
+        const long cap = (long)(p->ws_ext_bytes / per_sig);
+        if (cap < c) c = cap;
+    }
     if (c < 1) c = 1;
     return c < count ? c : count;
 }
@@ -46,14 +53,16 @@ size_t fafi_run_ws_bytes(const fafh_plan *p, long count) {
     switch (p->kind) {
     case PK_C2C_4S:
         if (p->rs) return 0;
-        return (size_t)chunk_signals(p->n, count) * p->n * 2 * sizeof(float);
+        return (size_t)chunk_signals(p, p->n, p->n * 8, count) * p->n * 2 * sizeof(float);
     case PK_R2C_4S:
     case PK_C2R_4S:
         /* with the single pass the workspace is needed only for rows that are
          * not 8-byte aligned, which fall back to four-step: keep its size */
         if (p->ct4)
-            return (size_t)chunk_signals(p->n / 2, count) * (p->n / 2) * 2 * sizeof(float);
-        return (size_t)chunk_signals(p->n / 2, count) * (p->n / 2) * 4 * sizeof(float);
+            return (size_t)chunk_signals(p, p->n / 2, p->n / 2 * 8, count) * (p->n / 2) * 2 *
+                   sizeof(float);
+        return (size_t)chunk_signals(p, p->n / 2, p->n / 2 * 16, count) * (p->n / 2) * 4 *
+               sizeof(float);
     case PK_CWT:
         return fafi_cwt_ws_bytes(p, count, 1);
     default:
@@ -95,7 +104,7 @@ int fafi_run_c2c(fafh_plan *p, const float *ir, const float *ii, long idist, flo
         if (p->rs)   /* one pass, no workspace */
             return krerr(fafk_rs_c2c(p->inverse, inter, p->twn, ir, ii, idist, orr, oi, odist,
                                      count, p->scale, rs_grid(), stream));
-        long cs = chunk_signals(n, count);
+        long cs = chunk_signals(p, n, n * 8, count);
         float *ws = fafi_workspace(p, (size_t)cs * n * 2 * sizeof(float));
         if (!ws) return -1;
         if (p->ct4) {
@@ -150,7 +159,7 @@ int fafi_run_r2c(fafh_plan *p, const float *x, long idist, float *orr, float *oi
     if (p->rs && fafk_rs_pair_ok(x, idist))   /* one pass, no workspace */
         return krerr(fafk_rs_r2c(inter, p->twn, p->post, x, idist, orr, oi, odist, count,
                                  p->scale, rs_grid(), stream));
-    long cs = chunk_signals(nc, count);
+    long cs = chunk_signals(p, nc, p->ct4 ? nc * 8 : nc * 16, count);
     if (p->ct4) {
         /* column pass, then the row pass with the Kiss post-pass fused */
         void *w = fafi_workspace(p, (size_t)cs * nc * 2 * sizeof(float));
@@ -208,7 +217,7 @@ int fafi_run_c2r(fafh_plan *p, const float *ir, const float *ii, long idist, flo
     if (p->rs && fafk_rs_pair_ok(y, odist))   /* one pass, no workspace */
         return krerr(fafk_rs_c2r(inter, p->twn, p->pre, ir, ii, idist, y, odist, count,
                                  p->scale, rs_grid(), stream));
-    long cs = chunk_signals(nc, count);
+    long cs = chunk_signals(p, nc, p->ct4 ? nc * 8 : nc * 16, count);
     if (p->ct4) {
         /* Kiss pre-pass fused into the column pass, real pairs out of the row pass */
         void *w = fafi_workspace(p, (size_t)cs * nc * 2 * sizeof(float));

@@ -378,7 +378,13 @@ void fafi_plan_free(fafh_plan *p) {
 void *fafi_workspace(fafh_plan *p, size_t bytes) {
     if (p->ws_ext) {
         if (p->ws_ext_bytes >= bytes) return p->ws_ext;
-        fafi_err("lent workspace holds %zu bytes, the run needs %zu", p->ws_ext_bytes, bytes);
+        if (fafi_noalloc())
+            fafi_err("faf_execute_batch_noalloc: the run needs %zu bytes of scratch, more than "
+                     "the %zu bytes of the static scratch; use a smaller count",
+                     bytes, p->ws_ext_bytes);
+        else
+            fafi_err("lent workspace holds %zu bytes, the run needs %zu", p->ws_ext_bytes,
+                     bytes);
         return NULL;
     }
     if (p->ws_bytes >= bytes) {

@@ -53,6 +53,27 @@ void faf_graph_destroy(faf_graph *g);
 int faf_execute_batch(const faf_transform *t, faf_buffer *out, const faf_buffer *in,
                       size_t count, size_t in_dist, size_t out_dist);
 
+/* ---- batched execute that allocates nothing ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same transform as faf_execute_batch, with the same
+ * kernels and the same results bit for bit, but the workspace that plans
+ * longer than one workgroup holds (four-step intermediates of FFT/RFFT
+ * lengths above 16384 points, long CWT segments) comes from a static
+ * device array of FAF_NOALLOC_SCRATCH_BYTES per GPU instead of a buffer
+ * the plan allocates.  Four-step batches are cut into chunks that fit it; a
+ * run whose single chunk needs more fails with an error naming both sizes.
+ * Every buffer (and a pipeline's bound spectrum) must be device memory.
+ * The call never allocates: anything that would (host staging, a plan
+ * record not yet resident) fails with a message instead, so run a plan once
+ * through faf_execute_batch (or create it) before calling this.
+ * All calls on one GPU share that GPU's scratch, which keeps the last
+ * call's contents: calls that overlap on one GPU (other threads, other
+ * streams) race and must be serialised by the caller. */
+#define FAF_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int faf_execute_batch_noalloc(const faf_transform *t, faf_buffer *out, const faf_buffer *in,
+                              size_t count, size_t in_dist, size_t out_dist);
+
 /* ---- ragged C2C batch in one persistent launch ----
  * Signals of different lengths are concatenated in split planes at
  * offsets = exclusive prefix sum of lengths.  Every length must be legal
