@@ -51,6 +51,47 @@ def clock_warmup(fn, torch, seconds):
             return {"launches": launches, "ms": round(el * 1e3, 1)}
 
 
+def c4_signal(torch, channels=64, length=1 << 22, fs=16000.0, seed=0, device="cuda"):
+    """The STFT workload's input: per channel the sum of 3 linear chirps
+    sweeping 50 -> 4000 Hz (or 4000 -> 50) over the record at fs = 16 kHz,
+    plus N(0, sigma = 0.1) noise, as float32 [channels, length].  Each chirp
+    has a seeded sweep direction, start phase and circular time offset (so
+    the three sit at different frequencies at any instant); phases are
+    computed in FP64 on the device.  Deterministic for a given seed."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    up = rng.integers(0, 2, (channels, 3))
+    phi = rng.uniform(0.0, 2.0 * np.pi, (channels, 3))
+    shift = rng.uniform(0.0, 1.0, (channels, 3))
+    T = length / fs
+    gen = torch.Generator(device=device).manual_seed(seed)
+    out = torch.empty((channels, length), dtype=torch.float32, device=device)
+    t = torch.arange(length, dtype=torch.float64, device=device) / fs
+    for c in range(channels):
+        acc = torch.zeros(length, dtype=torch.float64, device=device)
+        for j in range(3):
+            fa_, fb = (50.0, 4000.0) if up[c, j] else (4000.0, 50.0)
+            tau = torch.remainder(t + shift[c, j] * T, T)
+            acc += torch.sin(2.0 * np.pi * (fa_ * tau + (fb - fa_) * tau * tau / (2.0 * T)) +
+                             phi[c, j])
+        noise = torch.randn(length, generator=gen, device=device, dtype=torch.float32)
+        out[c] = (acc + 0.1 * noise.double()).float()
+    return out
+
+
+def rel_l2_max(np, got, truth):
+    """Largest relative L2 error over the rows of got against truth."""
+    got = np.atleast_2d(np.asarray(got, dtype=np.complex128))
+    truth = np.atleast_2d(np.asarray(truth, dtype=np.complex128))
+    e = [np.linalg.norm(g - t) / max(np.linalg.norm(t), 1e-300) for g, t in zip(got, truth)]
+    return float(max(e))
+
+
+def rows_of(torch, t, idx):
+    """Host float64 copies of rows idx of a device tensor."""
+    return torch.stack([t[i] for i in idx]).double().cpu().numpy()
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +178,14 @@ def main():
     torch.cuda.synchronize()
     rt = (torch.linalg.vector_norm(y - x) / torch.linalg.vector_norm(x)).item()
     extra["c2_4096_round_trip_rel_l2"] = rt
+    # per-config error against FP64 (numpy's double-precision FFT) on sampled
+    # rows: the first, middle and last signal of the batch
+    err = {}
+    idx = [0, b // 2, b - 1]
+    xs64 = rows_of(torch, x, idx)
+    X = rows_of(torch, re, idx) + 1j * rows_of(torch, im, idx)
+    err["rfft_4096"] = rel_l2_max(np, X, np.fft.rfft(xs64, axis=-1))
+    err["irfft_4096"] = rel_l2_max(np, rows_of(torch, y, idx), np.fft.irfft(X, n, axis=-1))
 
     # ---------------- CPU baseline: the reference's own CPU rfft (its sources
     # compiled into oracle/_ref: VM/JIT, AVX2), one plan per thread, on this
@@ -148,7 +197,15 @@ def main():
         if args.no_cpu:
             raise RuntimeError("skipped (--no-cpu)")
         import oracle as O
-        threads = min(16, len(os.sched_getaffinity(0)))
+        # every core this process may use, up to the host's declared CPU
+        # share for this job (OMP_NUM_THREADS; the GPU host sets 16 per GPU):
+        # threads beyond the share would only time-slice the same cores
+        share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = len(os.sched_getaffinity(0))
+        if share > 0:
+            threads = min(threads, share)
+        extra["cpu_host_nproc"] = os.cpu_count()
+        extra["cpu_threads_used"] = threads
         use_ref = O.ref_available()
         # output planes reused by every timing, so page faults of fresh
         # allocations stay out of the measurement
@@ -222,7 +279,7 @@ def main():
 
     if not args.quick:
         only = set(filter(None, args.only.split(",")))
-        run_extras(lib, fa, torch, np, timed, extra, only)
+        run_extras(lib, fa, torch, np, timed, extra, only, err)
 
     line = {
         "metric": "batched FFT GB/s (% HBM peak) vs plain launch; p50/p99 at mixed-size load",
@@ -242,6 +299,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "clock_warmup": clock_warm,
+        "err_vs_fp64": err,
         "extra": extra,
     }
     print(json.dumps(line))
@@ -263,7 +321,7 @@ def traffic_from_profile():
     return None
 
 
-def run_extras(lib, fa, torch, np, timed, extra, only):
+def run_extras(lib, fa, torch, np, timed, extra, only, err):
     def want(k):
         ok = not only or k in only
         if ok:
@@ -289,6 +347,11 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
             ms2 = timed(lambda: fa.check(lib.faf_execute_batch(ti, C.byref(by), C.byref(bo), b, 0, 0)), 20, 10)
             extra[f"r2c_{n}_gbs"] = round(ab / (ms * 1e-3) / 1e9, 1)
             extra[f"c2r_{n}_gbs"] = round(ab / (ms2 * 1e-3) / 1e9, 1)
+            idx = [0, b // 2, b - 1]
+            X = rows_of(torch, re, idx) + 1j * rows_of(torch, im, idx)
+            err[f"rfft_{n}"] = rel_l2_max(np, X, np.fft.rfft(rows_of(torch, x, idx), axis=-1))
+            err[f"irfft_{n}"] = rel_l2_max(np, rows_of(torch, y, idx),
+                                           np.fft.irfft(X, n, axis=-1))
             lib.faf_destroy_transform(t)
             lib.faf_destroy_transform(ti)
             del x, re, im, y
@@ -311,6 +374,10 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
         bo = fa.buffer_real(y.data_ptr(), n)
         ms = timed(lambda: fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0)), 20, 10)
         extra["notch_pipeline_4096_gbs"] = round(b * 8 * n / (ms * 1e-3) / 1e9, 1)
+        idx = [0, b // 2, b - 1]
+        xs64 = rows_of(torch, x, idx)
+        err["notch_4096"] = rel_l2_max(np, rows_of(torch, y, idx),
+                                       np.fft.irfft(np.fft.rfft(xs64, axis=-1) * m, n, axis=-1))
         lib.faf_destroy_transform(t)
         del x, y
         torch.cuda.empty_cache()
@@ -338,6 +405,9 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
         extra["c1_c2c_4096x256_gbs"] = round(b * 16 * n / (ms * 1e-3) / 1e9, 1)
         extra["c1_fwd_inv_4096x256_us"] = round(ms_fi * 1e3, 2)
         extra["c1_round_trip_rel_l2"] = rt
+        xc = xr.double().cpu().numpy() + 1j * xi.double().cpu().numpy()
+        err["c1_fft_4096x256"] = rel_l2_max(
+            np, yr.double().cpu().numpy() + 1j * yi.double().cpu().numpy(), np.fft.fft(xc, axis=-1))
 
         # the same forward+inverse captured once into a HIP graph and replayed
         fa.check(lib.faf_graph_begin(), "graph begin")
@@ -382,6 +452,19 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
         ms = timed(lambda: fa.check(lib.faf_ragged_execute(r, yr.data_ptr(), yi.data_ptr(), xr.data_ptr(), xi.data_ptr())), 50, 20)
         extra["c3_ragged_20000_ms"] = round(ms, 3)
         extra["c3_ragged_gbs"] = round(16 * tot / (ms * 1e-3) / 1e9, 1)
+        # every 50th signal and the first of each length, against FP64
+        off = np.concatenate([[0], np.cumsum(lens)])
+        pick = set(range(0, len(lens), 50))
+        for L_ in lens_set:
+            pick.add(int(np.nonzero(lens == L_)[0][0]))
+        hx = xr.double().cpu().numpy() + 1j * xi.double().cpu().numpy()
+        hy = yr.double().cpu().numpy() + 1j * yi.double().cpu().numpy()
+        worst = 0.0
+        for i in sorted(pick):
+            a_, e_ = off[i], off[i + 1]
+            worst = max(worst, rel_l2_max(np, hy[a_:e_], np.fft.fft(hx[a_:e_])))
+        err["c3_ragged_20000"] = worst
+        extra["c3_err_signals_checked"] = len(pick)
         lib.faf_ragged_destroy(r)
 
     # framed STFT: 64 channels x 2^22, Hann 1024, hop 256
@@ -391,12 +474,22 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
         t = lib.faf_create_stft(C.byref(cfg))
         nf = lib.faf_stft_frames(t, L)
         nb = win // 2 + 1
-        x = torch.randn((ch, L), device="cuda")
+        x = c4_signal(torch, ch, L)              # 3 chirps 50-4000 Hz + N(0, 0.1)
         re = torch.empty((ch, nf, nb), device="cuda")
         im = torch.empty((ch, nf, nb), device="cuda")
         ms = timed(lambda: fa.check(lib.faf_execute_stft(t, re.data_ptr(), im.data_ptr(), x.data_ptr(), L, ch, 0)), 10, 3)
         extra["stft_64x2^22_ms"] = round(ms, 3)
         extra["stft_gbs"] = round((4 * L * ch + 8 * nb * nf * ch) / (ms * 1e-3) / 1e9, 1)
+        extra["stft_input"] = "3 linear chirps 50-4000 Hz at 16 kHz + N(0, 0.1) noise, seed 0"
+        # channels 0 and 63: first, middle and last frame against FP64
+        w64 = 0.5 * (1.0 - np.cos(2.0 * np.pi * np.arange(win) / (win - 1)))
+        worst = 0.0
+        for c_ in (0, ch - 1):
+            for f_ in (0, nf // 2, nf - 1):
+                seg = x[c_, f_ * hop: f_ * hop + win].double().cpu().numpy()
+                got = (re[c_, f_].double() + 1j * im[c_, f_].double()).cpu().numpy()
+                worst = max(worst, rel_l2_max(np, got, np.fft.rfft(seg * w64)))
+        err["stft_64x2^22"] = worst
         lib.faf_destroy_transform(t)
         del x, re, im
         torch.cuda.empty_cache()
@@ -444,6 +537,27 @@ def run_extras(lib, fa, torch, np, timed, extra, only):
         ms = timed(lambda: fa.check(lib.faf_execute_cwt_complex(t, z.data_ptr(), x.data_ptr(), segs, 0)), 20, 5)
         extra["cwt_1024seg_ms"] = round(ms, 3)
         extra["cwt_gbs"] = round(segs * (4 * n + 8 * n * rows) / (ms * 1e-3) / 1e9, 1)
+        # first and last segment against FP64 analytic rows built from the
+        # bank's own filters: z_r = ifft(X * psi_r * (1, 2, ..., 2, 1, 0, ...))
+        nbn = n // 2 + 1
+        psi = np.zeros((rows, nbn))
+        if lib.faf_cwt_has_lowpass(t):
+            psi[0] = np.ctypeslib.as_array(lib.faf_cwt_phi_f32(t), (nbn,))
+        off_ = 1 if lib.faf_cwt_has_lowpass(t) else 0
+        for j in range(rows - off_):
+            psi[j + off_] = np.ctypeslib.as_array(lib.faf_cwt_psi_f32(t, j), (nbn,))
+        one = np.zeros(n)
+        one[0] = one[n // 2] = 1.0
+        one[1:n // 2] = 2.0
+        worst = 0.0
+        for s_ in (0, segs - 1):
+            X64 = np.fft.fft(x[s_].double().cpu().numpy())
+            H = np.zeros((rows, n))
+            H[:, :nbn] = psi
+            truth = np.fft.ifft(X64[None, :] * H * one[None, :], axis=-1)
+            got = z[s_].double().cpu().numpy()
+            worst = max(worst, rel_l2_max(np, got[..., 0] + 1j * got[..., 1], truth))
+        err["cwt_4096x64"] = worst
         lib.faf_destroy_transform(t)
         del x, z
         torch.cuda.empty_cache()

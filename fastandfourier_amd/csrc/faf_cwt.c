@@ -445,17 +445,22 @@ int fafi_cwt_build(fafh_plan *p, const faf_cwt_config *cfg, int inverse_kind) {
         p->sub_c2r = p->sub_c2c ? cwt_sub(n, 1, 1, p->scale) : NULL;
         return p->sub_c2r ? 0 : -1;
     }
-    /* the inverse holds a row spectrum and the accumulator side by side */
-    if (!fafi_radix_fits(nc, 0) || 2 * (nc + (nc >> 4) + 1) * 8 > FAFD_MAX_LDS_BYTES ||
-        fafi_make_fft(nc, 0, 0, NULL, 0, &p->a)) {
-        fafi_err("cwt: inverse of length %zu does not fit one workgroup", n);
-        return -1;
-    }
-    if (fafi_make_fft(nc, 1, 0, NULL, 0, &p->b)) return -1;
-    p->dual_dev = upload_f32(b->dual32, b->n_rows * b->stride);
     p->kind = PK_ICWT;
     p->scale = (float)(1.0 / (double)n);
-    return p->dual_dev ? 0 : -1;
+    p->dual_dev = upload_f32(b->dual32, b->n_rows * b->stride);
+    if (!p->dual_dev) return -1;
+    if (inverse_kind == FAF_CWT_INV_L1) return 0;
+    /* dual inverse: one kernel when a row spectrum and the accumulator fit
+     * one workgroup's LDS side by side */
+    if (fafi_radix_fits(nc, 0) && 2 * (nc + (nc >> 4) + 1) * 8 <= FAFD_MAX_LDS_BYTES &&
+        !fafi_make_fft(nc, 0, 0, NULL, 0, &p->a))
+        return fafi_make_fft(nc, 1, 0, NULL, 0, &p->b);
+    /* longer segments: one batched rfft over every row, the dual
+     * accumulation across rows, then a batched irfft (reference
+     * src/faf_cwt.c:958-980 does the same per row) */
+    p->sub_r2c = cwt_sub(n, 1, 0, 1.0f);
+    p->sub_c2r = p->sub_r2c ? cwt_sub(n, 1, 1, p->scale) : NULL;
+    return p->sub_c2r ? 0 : -1;
 }
 
 void fafi_cwt_free(fafh_plan *p) {
@@ -585,6 +590,58 @@ int fafi_run_cwt(fafh_plan *p, const float *x, long idist, float *out, long odis
     return rc;
 }
 
+/* long-segment dual inverse: segments per chunk, so the row spectra stay
+ * <= 256 MB; workspace = spectra, accumulators, then the sub-plans' own */
+static long icwt_chunk(const fafh_plan *p, long count) { return cwt_chunk(p, count); }
+
+static size_t icwt_ws_bytes(const fafh_plan *p, long count) {
+    const size_t nb = p->n / 2 + 1, rows = (size_t)p->rows_total;
+    const long c = icwt_chunk(p, count);
+    size_t yb = align256((size_t)c * rows * nb * 8), ab = align256((size_t)c * nb * 8);
+    size_t s1 = fafi_run_ws_bytes(p->sub_r2c, c * (long)rows);
+    size_t s2 = fafi_run_ws_bytes(p->sub_c2r, c);
+    return yb + ab + (s1 > s2 ? s1 : s2);
+}
+
+static int run_icwt_long(fafh_plan *p, const float *w, long idist, float *y, long odist,
+                         long count, void *stream) {
+    const long n = (long)p->n, nb = n / 2 + 1, rows = p->rows_total;
+    const long c = icwt_chunk(p, count);
+    const size_t need = icwt_ws_bytes(p, count);
+    char *ws = fafi_workspace(p, need);
+    if (!ws) return -1;
+    float *Y = (float *)ws;
+    const size_t yb = align256((size_t)c * (size_t)rows * (size_t)nb * 8);
+    float *A = (float *)(ws + yb);
+    const size_t ab = align256((size_t)c * (size_t)nb * 8);
+    char *sub = ws + yb + ab;
+    const size_t subb = need - yb - ab;
+    int rc = 0;
+    for (long s0 = 0; s0 < count; s0 += c) {
+        const long cc = count - s0 < c ? count - s0 : c;
+        if (idist == rows * n) {       /* the chunk's rows are one contiguous batch */
+            if (LENT(p->sub_r2c, sub, subb,
+                     fafi_run_r2c(p->sub_r2c, w + s0 * idist, n, Y, NULL, nb, cc * rows,
+                                  stream)))
+                return -1;
+        } else {
+            for (long s = 0; s < cc; ++s)
+                if (LENT(p->sub_r2c, sub, subb,
+                         fafi_run_r2c(p->sub_r2c, w + (s0 + s) * idist, n,
+                                      Y + 2 * s * rows * nb, NULL, nb, rows, stream)))
+                    return -1;
+        }
+        if (fafk_icwt_mulacc(Y, nb, (int)rows, p->dual_dev, p->psi_stride, A, cc, stream)) {
+            fafi_err("%s", fafk_last_error());
+            return -1;
+        }
+        if (LENT(p->sub_c2r, sub, subb,
+                 fafi_run_c2r(p->sub_c2r, A, NULL, nb, y + s0 * odist, odist, cc, stream)))
+            return -1;
+    }
+    return rc;
+}
+
 int fafi_run_icwt(fafh_plan *p, const float *w, long idist, float *y, long odist, long count,
                   void *stream) {
     fafi_cwt_bank *b = p->bank;
@@ -598,6 +655,7 @@ int fafi_run_icwt(fafh_plan *p, const float *w, long idist, float *y, long odist
         if (rc) fafi_err("%s", fafk_last_error());
         return rc;
     }
+    if (p->sub_r2c) return run_icwt_long(p, w, idist, y, odist, count, stream);
     int rc = fafk_icwt(&p->a, &p->b, p->post, p->pre, p->dual_dev, p->psi_stride,
                        p->rows_total, w, idist, y, odist, count, p->scale, stream);
     if (rc) fafi_err("%s", fafk_last_error());

@@ -103,6 +103,10 @@ int fafk_cwt(const fafd_fft *Ph, const fafd_fft *Pn, int real_out, const void *p
 int fafk_icwt(const fafd_fft *Pf, const fafd_fft *Pi, const void *post, const void *pre,
               const float *dual, long dual_stride, int rows, const float *w, long in_dist,
               float *y, long out_dist, long count, float scale, void *stream);
+/* long-segment dual inverse CWT: acc = sum_r Y_r * dual_r over interleaved
+ * row spectra Y [count][rows][nb]; acc interleaved [count][nb] */
+int fafk_icwt_mulacc(const float *Y, long nb, int rows, const float *dual, long stride,
+                     float *acc, long count, void *stream);
 /* L1 one-integral inverse CWT over real rows [lowpass?][n_scales wavelet rows] */
 int fafk_icwt_l1(const float *w, long in_dist, float *y, long out_dist, long n, int n_scales,
                  int has_lp, float dlog, float c, long count, void *stream);

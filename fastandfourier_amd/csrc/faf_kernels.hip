@@ -530,6 +530,44 @@ extern "C" int fafk_icwt_l1(const float *w, long in_dist, float *y, long out_dis
     return 0;
 }
 
+/* Dual-frame accumulation of the long-segment inverse CWT (reference
+ * src/faf_cwt.c:958-977): acc[k] = sum over rows r, in row order, of
+ * Y_r[k] * dual_r[k], with the imaginary parts of DC and Nyquist cleared.
+ * Y: interleaved row spectra [count][rows][nb]; acc: interleaved [count][nb].
+ * One thread per bin; a row's bins are read contiguously by a wave. */
+__global__ __launch_bounds__(256) void k_icwt_mulacc(const float2 *__restrict__ Y, long nb,
+                                                     int rows, const float *__restrict__ dual,
+                                                     long stride, float2 *__restrict__ acc,
+                                                     long bps) {
+    const long seg = blockIdx.x / bps;
+    const long k = (blockIdx.x - seg * bps) * 256 + threadIdx.x;
+    if (k >= nb) return;
+    const float2 *y = Y + seg * rows * nb + k;
+    float ar = 0.0f, ai = 0.0f;
+    for (int r = 0; r < rows; ++r) {
+        const float2 v = y[(long)r * nb];
+        const float d = dual[(long)r * stride + k];
+        ar = fmaf(v.x, d, ar);
+        ai = fmaf(v.y, d, ai);
+    }
+    if (k == 0 || k == nb - 1) ai = 0.0f;
+    acc[seg * nb + k] = make_float2(ar, ai);
+}
+
+extern "C" int fafk_icwt_mulacc(const float *Y, long nb, int rows, const float *dual,
+                                long stride, float *acc, long count, void *stream) {
+    const long bps = (nb + 255) / 256;
+    if (nb <= 0 || rows <= 0 || count <= 0 || stride < nb || bps * count > 0x7fffffffL) {
+        snprintf(g_err, sizeof g_err, "k_icwt_mulacc: unsupported shape nb=%ld rows=%d count=%ld",
+                 nb, rows, count);
+        return -1;
+    }
+    hipLaunchKernelGGL(k_icwt_mulacc, dim3((unsigned)(bps * count)), dim3(256), 0, S(stream),
+                       (const float2 *)Y, nb, rows, dual, stride, (float2 *)acc, bps);
+    LAUNCH_CHECK("k_icwt_mulacc");
+    return 0;
+}
+
 extern "C" int fafk_fs_col(const fafd_fft *P1, int inv, int in_kind, int n2, int cols,
                            const void *twn, const float *in_re, const float *in_im,
                            long in_dist, float *t_re, float *t_im, long t_dist, long count,

@@ -240,6 +240,28 @@ def icwt_l1_f32(w, n_scales, has_lp, voices, c):
     return y
 
 
+def icwt_dual64(w, psi):
+    """Dual-frame inverse CWT in FP64 (reference src/faf_cwt.c:441-470 for the
+    dual table, :958-980 for the inverse): dual_r[k] = psi_r[k] / (sum_r
+    psi_r[k]^2 + 1e-6), sums over every row of the live FP32 table psi
+    [rows, >= n/2+1] (lowpass first when present); acc = sum_r rfft(w_r) *
+    dual_r in row order; imag of DC and Nyquist cleared; y = irfft(acc)
+    with the 1/n of the inverse.  w: real rows [rows, n]."""
+    w = np.asarray(w, dtype=np.float64)
+    rows, n = w.shape
+    nb = n // 2 + 1
+    p = np.asarray(psi, dtype=np.float32)[:rows, :nb].astype(np.float64)
+    ssum = (p * p).sum(axis=0)
+    dual = np.where(ssum > 0.0, p / (ssum + 1e-6), 0.0)
+    acc = np.zeros(nb, np.complex128)
+    for r in range(rows):
+        acc += fft64(w[r])[0][:nb] * dual[r]
+    acc[0] = acc[0].real
+    acc[nb - 1] = acc[nb - 1].real
+    full = np.concatenate([acc, np.conj(acc[1:nb - 1][::-1])])
+    return fft64(full, inverse=True)[0].real / n
+
+
 def _out_planes(cnt, nb, out):
     """re/im planes for cnt spectra: `out` (a pair of float32 arrays of at
     least cnt*nb values, reused across timings) or new ones"""
@@ -294,6 +316,8 @@ def ref():
         sig = {
             "faf_create_fft": (PT, [C.POINTER(fa.FafConfig)]),
             "faf_create_rfft": (PT, [C.POINTER(fa.FafConfig)]),
+            "faf_create_stft": (PT, [C.POINTER(fa.FafConfig)]),
+            "faf_execute_vm": (C.c_int, [PT, C.c_void_p, C.c_void_p]),
             "faf_create_inverse": (PT, [PT]),
             "faf_execute": (C.c_int, [PT, C.POINTER(fa.FafBuffer), C.POINTER(fa.FafBuffer)]),
             "faf_destroy_transform": (None, [PT]),

@@ -102,6 +102,80 @@ def ref_cwt(n, fs, voices=None, fmin=None, fmax=None):
     return t, psi, freqs, scales, report
 
 
+def ref_icwt_dual(n, fs, rows_in, voices=None, fmin=None, fmax=None):
+    """The reference's dual-frame inverse (FAF_CWT_INV_DUAL) of real rows."""
+    h = O.ref()
+    cc = h.faf_cwt_config_init(n)
+    cc.fs = fs
+    if voices:
+        cc.voices = voices
+    if fmin:
+        cc.fmin = fmin
+    if fmax:
+        cc.fmax = fmax
+    cc.backend = 1
+    ti = h.faf_create_icwt(C.byref(cc), 0)
+    assert ti, h.faf_get_error()
+    w = np.ascontiguousarray(rows_in, dtype=np.float32)
+    y = np.zeros(n, np.float32)
+    bi = fa.buffer_real(w.ctypes.data, n)
+    bo = fa.buffer_real(y.ctypes.data, n)
+    assert h.faf_execute(ti, C.byref(bo), C.byref(bi)) == 0, h.faf_get_error()
+    h.faf_destroy_transform(ti)
+    return y
+
+
+def ref_cwt_rows(t, x):
+    h = O.ref()
+    n = x.size
+    rows = h.faf_cwt_n_rows(t)
+    w = np.zeros(rows * n, np.float32)
+    bi = fa.buffer_real(x.ctypes.data, n)
+    bo = fa.buffer_real(w.ctypes.data, rows * n)
+    assert h.faf_execute(t, C.byref(bo), C.byref(bi)) == 0, h.faf_get_error()
+    return w.reshape(rows, n)
+
+
+def add_dual_and_serving_fixtures(fx):
+    """Fixtures added after the first set, from their own generator so the
+    earlier ones keep their bytes:
+      * the reference's dual-frame inverse of the n=512 default-bank rows,
+        and of the C4 bank's rows of one 4096-sample segment;
+      * the serving bank (n=1024, fs=16 kHz, 8 voices, 32 scales from
+        fmin = 16 fs/n) tables, frequencies and report;
+      * the reference's single-frame windowed STFT (interleaved, VM path)
+        for n=1000, win 800."""
+    rng = np.random.default_rng(100)
+    h = O.ref()
+    fx["cwt512_dual_out"] = ref_icwt_dual(512, 1.0, fx["cwt512_rows"])
+    f0 = 62.5
+    t, psi, _, _, _ = ref_cwt(4096, 16000.0, 11, f0, f0 * 2 ** (63.5 / 11))
+    x = rng.standard_normal(4096).astype(np.float32)
+    w = ref_cwt_rows(t, x)
+    h.faf_destroy_transform(t)
+    fx["cwt4096_seg_in"] = x
+    fx["cwt4096_seg_rows"] = w
+    fx["cwt4096_dual_out"] = ref_icwt_dual(4096, 16000.0, w, 11, f0, f0 * 2 ** (63.5 / 11))
+    n, fs = 1024, 16000.0
+    fmin = 16 * fs / n
+    t, psi, freqs, scales, rep = ref_cwt(n, fs, 8, fmin, fmin * 2 ** (31.5 / 8))
+    h.faf_destroy_transform(t)
+    fx["cwt1024v8_psi"] = psi
+    fx["cwt1024v8_freqs"] = freqs
+    fx["cwt1024v8_report"] = rep
+    n, win = 1000, 800
+    cfg = fa.config_init(n, layout="interleaved", win_length=win, backend=1)
+    ts = h.faf_create_stft(C.byref(cfg))
+    assert ts, h.faf_get_error()
+    z = uniform_c(rng, 1, n)[0]
+    zin = np.ascontiguousarray(z.view(np.float32))
+    zout = np.zeros(2 * n, np.float32)
+    assert h.faf_execute_vm(ts, zout.ctypes.data, zin.ctypes.data) == 0, h.faf_get_error()
+    h.faf_destroy_transform(ts)
+    fx["stft1_1000_in"] = z
+    fx["stft1_1000_out"] = zout.view(np.complex64)
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/libfaf_ref.so missing: run make -C oracle ref")
@@ -167,6 +241,7 @@ def main():
     h.faf_destroy_transform(ti)
     fx["cwt512_l1_out"] = y
     fx["cwt512_voices"] = np.array([cc.voices])
+    add_dual_and_serving_fixtures(fx)
     out = os.path.join(HERE, "golden.npz")
     np.savez_compressed(out, **fx)
     print(f"wrote {out}: {len(fx)} arrays, {os.path.getsize(out) / 1e6:.2f} MB")

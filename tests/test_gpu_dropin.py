@@ -22,6 +22,28 @@ def build(out):
                     "-o", out], check=True)
 
 
+REF_EXAMPLES = "/root/reference/examples"
+# the reference's own example programs that stay on the transform path
+EXAMPLES = ["simple_rfft", "simple_fft", "spectral_notch", "fft_convolve", "cwt_filterbank",
+            "cwt_reconstruct", "split_corr", "analytic_envelope", "matched_filter"]
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="reference tree not present")
+@pytest.mark.parametrize("name", EXAMPLES)
+def test_reference_example_links_unchanged(tmp_path, name):
+    """CPU: the reference's own example program, compiled from where it lies
+    in the reference tree against include/ (nothing is copied), links
+    against this library with no change."""
+    cc = shutil.which("cc") or shutil.which("gcc")
+    if not cc:
+        pytest.skip("no C compiler")
+    src = os.path.join(REF_EXAMPLES, name + ".c")
+    r = subprocess.run([cc, "-std=c11", "-O1", src, "-I", os.path.join(ROOT, "include"),
+                        "-L", LIBDIR, "-lfastandfourier_amd", f"-Wl,-rpath,{LIBDIR}", "-lm",
+                        "-o", str(tmp_path / name)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_dropin_program_links(tmp_path):
     """CPU: the reference-API program compiles and links against this library."""
     build(str(tmp_path / "dropin"))

@@ -1,0 +1,246 @@
+"""Every configuration at its full size on the GPU, checked on sampled signals.
+
+The other parity tests run each configuration at reduced size; these run the
+shapes the benchmark times and check sampled outputs against the oracle's
+FP64 truth and, where the compiled reference (oracle/_ref) is present,
+against the reference library's own FP32 output on the same input.  Each
+case prints one "ERR" line (configuration, worst relative L2 against FP64,
+worst against the reference's FP32) so the per-configuration error can be
+read off the log.  Bar: relative L2 <= 1e-6 * log2(N) against FP64.
+
+  C2  rfft/irfft N = 1024, 4096, 16384, 65536, batch 2^28/N (1 GiB in):
+      rows 0, b/2, b-1;
+  C2  the notch pipeline at N=4096, batch 65536: rows 0, b/2, b-1;
+  C3  20,000 ragged signals (lengths drawn with seed 0 as in the benchmark):
+      every 50th signal and the first of each length;
+  C4  STFT of 64 x 2^22 samples of the chirp-plus-noise input (win 1024,
+      hop 256): channels 0 and 63, first / middle / last frame (channel
+      63's last output rows lie past 2^31 bytes into each plane);
+  C4  CWT of 1024 segments of 4096 with 64 scales + lowpass: the first and
+      the last segment, against the oracle's FP64 analytic rows built from
+      the reference's own filter table (tests/golden).
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import bench
+import fastandfourier_amd as fa
+import oracle as O
+from conftest import rel_l2, tol
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import gpu_util as G  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def worst(pairs):
+    return max(rel_l2(a, b) for a, b in pairs)
+
+
+def report(name, e64, eref):
+    say(f"ERR {name} vs_fp64={e64:.3e} vs_ref_fp32={'n/a' if eref is None else f'{eref:.3e}'}")
+
+
+@pytest.mark.parametrize("n", [1024, 4096, 16384, 65536])
+def test_c2_full_batch_sampled(n):
+    say(f"case: full-size rfft/irfft n={n}")
+    lib = G.L()
+    b = (1 << 28) // n
+    nb = n // 2 + 1
+    gen = torch.Generator(device="cuda").manual_seed(n)
+    x = torch.randn((b, n), generator=gen, device="cuda")
+    re, im = G.empty((b, nb)), G.empty((b, nb))
+    y = G.empty((b, n))
+    t = G.create("rfft", n)
+    ti = lib.faf_create_inverse(t)
+    try:
+        bi = fa.buffer_real(x.data_ptr(), n)
+        bo = fa.buffer_hermitian(re.data_ptr(), im.data_ptr(), nb)
+        by = fa.buffer_real(y.data_ptr(), n)
+        fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0), "rfft")
+        fa.check(lib.faf_execute_batch(ti, C.byref(by), C.byref(bo), b, 0, 0), "irfft")
+        torch.cuda.synchronize()
+        rows = [0, b // 2, b - 1]
+        xs = x[rows].cpu().numpy()
+        X = re[rows].cpu().numpy() + 1j * im[rows].cpu().numpy()
+        ys = y[rows].cpu().numpy()
+        tX = [O.fft64(v.astype(np.float64))[0][:nb] for v in xs]
+        e64 = worst(zip(X, tX))
+        # irfft against FP64 of the same bins; and the round trip
+        ty = [np.fft.irfft(v.astype(np.complex128), n) for v in X]
+        e64i = worst(zip(ys, ty))
+        eref = erefi = None
+        if O.ref_available():
+            eref = worst(zip(X, O.ref_rfft(xs)))
+            erefi = worst(zip(ys, O.ref_irfft(X.astype(np.complex64), n)))
+        report(f"rfft_{n}x{b}", e64, eref)
+        report(f"irfft_{n}x{b}", e64i, erefi)
+        assert e64 < tol(n) and e64i < tol(n)
+        if eref is not None:
+            assert eref < tol(n) and erefi < tol(n)
+        assert rel_l2(ys, xs) < 2 * tol(n)
+        # DC and Nyquist bins are real
+        assert np.all(im[rows, 0].cpu().numpy() == 0) and np.all(im[rows, nb - 1].cpu().numpy() == 0)
+    finally:
+        G.destroy(ti)
+        G.destroy(t)
+
+
+def test_c2_notch_full_batch_sampled():
+    say("case: full-size notch pipeline 4096 x 65536")
+    lib = G.L()
+    n = 4096
+    b = (1 << 28) // n
+    nb = n // 2 + 1
+    m = np.ones(nb, np.float32)
+    m[-(-n // 10): n // 8] = 0.0
+    t = lib.chirp_compile(b"(pipeline (rfft :size 4096) (mul-spectrum) (irfft))")
+    assert t, lib.faf_get_error()
+    hr = G.dev(m)
+    hi = torch.zeros_like(hr)
+    fa.check(lib.chirp_bind(t, b"H", hr.data_ptr(), hi.data_ptr(), nb), "bind")
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    x = torch.randn((b, n), generator=gen, device="cuda")
+    y = torch.empty_like(x)
+    try:
+        bi = fa.buffer_real(x.data_ptr(), n)
+        bo = fa.buffer_real(y.data_ptr(), n)
+        fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0), "notch")
+        torch.cuda.synchronize()
+        rows = [0, b // 2, b - 1]
+        xs = x[rows].cpu().numpy()
+        ys = y[rows].cpu().numpy()
+        t64 = [np.fft.irfft(O.fft64(v.astype(np.float64))[0][:nb] * m, n) for v in xs]
+        e64 = worst(zip(ys, t64))
+        e32 = worst(zip(ys, O.pipeline32(xs, n, [2], h=m.astype(np.complex64))))
+        report(f"notch_{n}x{b}", e64, None)
+        say(f"  against the oracle's FP32 restatement of the reference pipeline: {e32:.3e}")
+        assert e64 < tol(n) and e32 < tol(n)
+        # the notched band is empty in the output spectrum
+        Y = np.fft.rfft(ys.astype(np.float64), axis=-1)
+        band = np.abs(Y[:, -(-n // 10): n // 8]).max()
+        assert band < 1e-3 * np.abs(Y).max()
+    finally:
+        lib.faf_destroy_transform(t)
+
+
+def test_c3_ragged_20000_sampled():
+    say("case: full-size ragged 20000")
+    lib = G.L()
+    lens_set = [60, 360, 1000, 3000, 4800, 11, 17, 31, 97, 8192, 1009]
+    lens = np.random.default_rng(0).choice(lens_set, size=20000)
+    arr = (C.c_size_t * len(lens))(*[int(v) for v in lens])
+    r = lib.faf_ragged_create(arr, len(lens), 0)
+    assert r, lib.faf_get_error()
+    try:
+        tot = lib.faf_ragged_total(r)
+        assert tot == int(lens.sum())
+        rng = np.random.default_rng(1)
+        xr = rng.uniform(-1, 1, tot).astype(np.float32)
+        xi = rng.uniform(-1, 1, tot).astype(np.float32)
+        dxr, dxi = G.dev(xr), G.dev(xi)
+        yr, yi = G.empty(tot), G.empty(tot)
+        fa.check(lib.faf_ragged_execute(r, yr.data_ptr(), yi.data_ptr(), dxr.data_ptr(),
+                                        dxi.data_ptr()), "ragged")
+        y = G.host(yr) + 1j * G.host(yi)
+        off = np.concatenate([[0], np.cumsum(lens)])
+        pick = set(range(0, len(lens), 50))
+        for L_ in lens_set:
+            pick.add(int(np.nonzero(lens == L_)[0][0]))
+        e64 = eref = 0.0
+        per_len = {}
+        for i in sorted(pick):
+            a, e = off[i], off[i + 1]
+            x = (xr[a:e] + 1j * xi[a:e]).astype(np.complex64)
+            el = rel_l2(y[a:e], O.fft64(x.astype(np.complex128))[0])
+            assert el < tol(lens[i]), f"signal {i} n={lens[i]}: {el}"
+            per_len[int(lens[i])] = max(per_len.get(int(lens[i]), 0.0), el)
+            e64 = max(e64, el)
+            if O.ref_available() and (i % 500 == 0 or i in (0,)):
+                eref = max(eref, rel_l2(y[a:e], O.ref_fft(x)[0]))
+        report("c3_ragged_20000", e64, eref if O.ref_available() else None)
+        say("  worst per length: " + ", ".join(f"{k}:{v:.2e}" for k, v in sorted(per_len.items())))
+        say(f"  signals checked: {len(pick)}")
+        if O.ref_available():
+            assert eref < tol(8192)
+    finally:
+        lib.faf_ragged_destroy(r)
+
+
+def test_c4_stft_64x2p22_chirps_sampled():
+    say("case: full-size STFT 64 x 2^22 chirps + noise")
+    lib = G.L()
+    win, hop, L, ch = 1024, 256, 1 << 22, 64
+    t = G.create("stft", win, layout="real", win_length=win, hop_length=hop)
+    try:
+        nf = lib.faf_stft_frames(t, L)
+        assert nf == (L - win) // hop + 1 == 16381
+        nb = win // 2 + 1
+        x = bench.c4_signal(torch, ch, L)
+        re = G.empty((ch, nf, nb))
+        im = G.empty((ch, nf, nb))
+        # the last channel's last frames lie past 2^31 bytes into each plane
+        assert ((ch - 1) * nf + nf - 2) * nb * 4 > (1 << 31)
+        fa.check(lib.faf_execute_stft(t, re.data_ptr(), im.data_ptr(), x.data_ptr(), L, ch, 0),
+                 "stft")
+        torch.cuda.synchronize()
+        e64 = 0.0
+        eref = 0.0
+        for c_ in (0, ch - 1):
+            for f_ in (0, 1, nf // 2, nf - 2, nf - 1):
+                seg = x[c_, f_ * hop: f_ * hop + win].cpu().numpy()
+                got = (re[c_, f_].cpu().numpy() + 1j * im[c_, f_].cpu().numpy())
+                e64 = max(e64, rel_l2(got, O.stft64(seg, win, hop)[0]))
+                eref = max(eref, rel_l2(got, O.stft32(seg, win, hop)[0]))
+        report("stft_64x2^22", e64, None)
+        say(f"  against the oracle's FP32 restatement (Hann x frame -> reference rfft): {eref:.3e}")
+        assert e64 < tol(win) and eref < tol(win)
+        # the chirps dominate: each checked frame's peak sits inside 50..4000 Hz
+        pk = np.argmax(np.abs(re[0, nf // 2].cpu().numpy() + 1j * im[0, nf // 2].cpu().numpy()))
+        assert 50.0 / 16000 * win - 2 <= pk <= 4000.0 / 16000 * win + 2
+    finally:
+        G.destroy(t)
+
+
+def test_c4_cwt_1024_segments_sampled(golden):
+    say("case: full-size CWT 1024 x 4096, 64 scales + lowpass")
+    lib = G.L()
+    n, segs = 4096, 1024
+    cc = lib.faf_cwt_config_init(n)
+    cc.fs = 16000.0
+    cc.voices = 11
+    cc.fmin = 62.5
+    cc.fmax = 62.5 * 2 ** (63.5 / 11)
+    t = lib.faf_create_cwt(C.byref(cc))
+    assert t, lib.faf_get_error()
+    try:
+        assert lib.faf_cwt_n_scales(t) == 64
+        rows = lib.faf_cwt_n_rows(t)
+        gen = torch.Generator(device="cuda").manual_seed(3)
+        x = torch.randn((segs, n), generator=gen, device="cuda")
+        z = G.empty((segs, rows, n, 2))
+        fa.check(lib.faf_execute_cwt_complex(t, z.data_ptr(), x.data_ptr(), segs, 0), "cwt")
+        torch.cuda.synchronize()
+        psi = golden["cwt4096_psi"]                  # the reference's own table
+        stride = psi.shape[1]
+        e64 = 0.0
+        for s_ in (0, segs - 1):
+            got = z[s_].cpu().numpy()
+            got = got[..., 0] + 1j * got[..., 1]
+            truth = O.cwt_complex64(x[s_].cpu().numpy(), psi, stride)
+            e64 = max(e64, rel_l2(got, truth))
+            # Re(row) is the reference's real row
+            assert rel_l2(got.real, O.cwt_real32(x[s_].cpu().numpy(), psi, stride)) < tol(n)
+        report(f"cwt_{n}x{segs}_64scales", e64, None)
+        assert e64 < tol(n)
+    finally:
+        lib.faf_destroy_transform(t)

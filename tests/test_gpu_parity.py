@@ -933,3 +933,218 @@ def test_autotune_keeps_results():
         assert rel_l2(after, before) < tol(n)
     finally:
         G.destroy(t)
+
+
+# ------------------------------------------- faf_execute_f32 and single-frame STFT
+
+@pytest.mark.parametrize("n", [1000, 4096])
+def test_execute_f32_interleaved_wrapper(n):
+    """faf_execute_f32 (reference src/faf_vm.c:976-1083): interleaved complex
+    in/out for FFT/IFFT, REAL -> interleaved n/2+1 bins for RFFT and back for
+    IRFFT; one signal, on device pointers and on host pointers."""
+    say(f"case: faf_execute_f32 n={n}")
+    lib = G.L()
+    rng = np.random.default_rng(n + 5)
+    x = cplx(rng, 1, n)[0]
+    xr = rng.standard_normal(n).astype(np.float32)
+    nb = n // 2 + 1
+    t = G.create("fft", n, layout="interleaved")
+    ti = G.create("fft", n, layout="interleaved", dir="inverse")
+    r = G.create("rfft", n, layout="interleaved")
+    ri = lib.faf_create_inverse(r)
+    assert ri, lib.faf_get_error()
+    try:
+        for on_dev in (True, False):
+            xin = x.view(np.float32).copy()
+            if on_dev:
+                a, y, z = G.dev(xin), G.empty(2 * n), G.empty(2 * n)
+                pa, py, pz = a.data_ptr(), y.data_ptr(), z.data_ptr()
+            else:
+                a, y, z = xin, np.empty(2 * n, np.float32), np.empty(2 * n, np.float32)
+                pa, py, pz = a.ctypes.data, y.ctypes.data, z.ctypes.data
+            fa.check(lib.faf_execute_f32(t, py, pa), "f32 fft")
+            fa.check(lib.faf_execute_f32(ti, pz, py), "f32 ifft")
+            yh = (G.host(y) if on_dev else y).view(np.complex64)
+            zh = (G.host(z) if on_dev else z).view(np.complex64)
+            assert rel_l2(yh, truth_fft(x[None])[0]) < tol(n), on_dev
+            assert rel_l2(zh, x) < 2 * tol(n), on_dev
+            # real: REAL n in, interleaved bins out, and back
+            if on_dev:
+                a2, X, w = G.dev(xr), G.empty(2 * nb), G.empty(n)
+                p2, pX, pw = a2.data_ptr(), X.data_ptr(), w.data_ptr()
+            else:
+                a2, X, w = xr.copy(), np.empty(2 * nb, np.float32), np.empty(n, np.float32)
+                p2, pX, pw = a2.ctypes.data, X.ctypes.data, w.ctypes.data
+            fa.check(lib.faf_execute_f32(r, pX, p2), "f32 rfft")
+            fa.check(lib.faf_execute_f32(ri, pw, pX), "f32 irfft")
+            Xh = (G.host(X) if on_dev else X).view(np.complex64)
+            wh = G.host(w) if on_dev else w
+            assert rel_l2(Xh, O.fft64(xr.astype(np.float64))[0][:nb]) < tol(n), on_dev
+            assert rel_l2(wh, xr) < 2 * tol(n), on_dev
+        # a SPLIT transform is refused by the interleaved wrapper
+        s_ = G.create("fft", n)
+        try:
+            assert lib.faf_execute_f32(s_, py, pa) != 0
+            assert b"INTERLEAVED" in lib.faf_get_error()
+        finally:
+            G.destroy(s_)
+    finally:
+        for h in (t, ti, r, ri):
+            G.destroy(h)
+
+
+@pytest.mark.parametrize("n,win", [(1000, 800), (1024, 1024), (4096, 3000)])
+def test_single_frame_stft_reference_semantics(n, win):
+    """faf_create_stft with INTERLEAVED layout keeps the reference's meaning
+    (src/faf_core.c:994-1032, 1369-1383): one forward C2C of length n of the
+    input times a symmetric Hann window of win_length samples, zero past it;
+    dir is forced forward; batched rows each get the same window."""
+    say(f"case: single-frame stft n={n} win={win}")
+    lib = G.L()
+    rng = np.random.default_rng(win)
+    x = cplx(rng, 3, n)
+    w = np.zeros(n, np.float64)
+    w[:win] = 0.5 * (1.0 - np.cos(2.0 * np.pi * np.arange(win) / (win - 1)))
+    t = G.create("stft", n, layout="interleaved", win_length=win, dir="inverse")
+    try:
+        assert lib.faf_transform_name(C.cast(t, C.POINTER(fa.FafTransform)).contents.type) \
+            == b"stft"
+        y = G.c2c(t, x, inter=True)
+        truth = O.fft64((x.astype(np.complex128) * w[None, :]))
+        assert rel_l2(y, truth) < tol(n)
+        # the symmetric Hann the window comes from matches the oracle's
+        assert np.allclose(O.hann32(win), w[:win].astype(np.float32), atol=1e-7)
+        y1 = G.c2c(t, x[1:2], inter=True)
+        assert np.array_equal(y1[0], y[1])                      # alone == in the batch
+    finally:
+        G.destroy(t)
+
+
+# ------------------------------------------------------------ dual-frame ICWT
+
+def _icwt_create(n, fs, voices=None, fmin=None, fmax=None):
+    lib = G.L()
+    cc = lib.faf_cwt_config_init(n)
+    cc.fs = fs
+    if voices:
+        cc.voices = voices
+    if fmin:
+        cc.fmin = fmin
+    if fmax:
+        cc.fmax = fmax
+    t = lib.faf_create_icwt(C.byref(cc), 0)
+    assert t, lib.faf_get_error()
+    return t
+
+
+def _run_icwt(t, w, count=1, in_dist=0):
+    """w: host float32 rows [count][rows][n] (or with in_dist padding)."""
+    lib = G.L()
+    n = t_n = C.cast(t, C.POINTER(fa.FafTransform)).contents.n
+    rows = lib.faf_cwt_n_rows(t)
+    dw = G.dev(np.ascontiguousarray(w, dtype=np.float32).ravel())
+    y = G.empty((count, n))
+    torch.cuda.synchronize()
+    bi = fa.buffer_real(dw.data_ptr(), rows * t_n)
+    bo = fa.buffer_real(y.data_ptr(), n)
+    fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), count, in_dist, 0), "icwt")
+    return G.host(y)
+
+
+@pytest.mark.parametrize("n", [512, 4096])
+def test_icwt_dual_matches_reference(golden, n):
+    """The device dual-frame inverse of the reference's own rows against the
+    compiled reference's FP32 dual inverse of the same rows and the oracle's
+    FP64 dual inverse (bar 1e-6 * log2 n)."""
+    say(f"case: icwt dual vs reference n={n}")
+    if n == 512:
+        t = _icwt_create(512, 1.0)
+        w, psi, ref = golden["cwt512_rows"], golden["cwt512_psi"], golden["cwt512_dual_out"]
+    else:
+        f0 = 62.5
+        t = _icwt_create(4096, 16000.0, 11, f0, f0 * 2 ** (63.5 / 11))
+        w, psi, ref = (golden["cwt4096_seg_rows"], golden["cwt4096_psi"],
+                       golden["cwt4096_dual_out"])
+    try:
+        y = _run_icwt(t, w)[0]
+        assert rel_l2(y, ref) < tol(n)
+        assert rel_l2(y, O.icwt_dual64(w, psi)) < tol(n)
+    finally:
+        G.L().faf_destroy_transform(t)
+
+
+@pytest.mark.parametrize("n", [32768, 65536])
+def test_icwt_dual_long_segments(n):
+    """Segments longer than one workgroup holds: batched row rffts, the dual
+    accumulation kernel and a batched irfft.  Device forward rows of two
+    segments go back through the device dual inverse; checked against the
+    oracle's FP64 dual inverse of the same rows and as a reconstruction
+    (reference tests/test_cwt.cpp:348-399, <= 1e-4 in FP32); also with a
+    padded segment distance (rows not one contiguous batch)."""
+    say(f"case: icwt dual long n={n}")
+    lib = G.L()
+    fs = 16000.0
+    tf = _cwt_create(n, fs)
+    ti = lib.faf_create_inverse(tf)
+    assert ti, lib.faf_get_error()
+    try:
+        rows = lib.faf_cwt_n_rows(tf)
+        tab, stride = _table(tf, lib)
+        rng = np.random.default_rng(n)
+        x = rng.standard_normal((2, n)).astype(np.float32)
+        dx = G.dev(x)
+        wd = G.empty((2, rows, n))
+        torch.cuda.synchronize()
+        bi = fa.buffer_real(dx.data_ptr(), n)
+        bo = fa.buffer_real(wd.data_ptr(), rows * n)
+        fa.check(lib.faf_execute_batch(tf, C.byref(bo), C.byref(bi), 2, 0, 0), "cwt rows")
+        w = G.host(wd)
+        y = _run_icwt(ti, w, count=2)
+        for s in range(2):
+            assert rel_l2(y[s], O.icwt_dual64(w[s], tab)) < tol(n), s
+            assert rel_l2(y[s], x[s]) < 1e-4, s
+        # padded distance between segments: per-segment row batches
+        pad = rows * n + 64
+        wp = np.zeros((2, pad), np.float32)
+        wp[:, :rows * n] = w.reshape(2, -1)
+        yp = _run_icwt(ti, wp, count=2, in_dist=pad)
+        assert np.array_equal(yp, y)
+    finally:
+        lib.faf_destroy_transform(tf)
+        lib.faf_destroy_transform(ti)
+
+
+def test_serving_bank_matches_reference(golden):
+    """The bank the serving loop builds for CWT requests (n=1024, fs=16 kHz,
+    8 voices, 32 scales from 16 fs/n) against the compiled reference's
+    tables, frequencies and report."""
+    say("case: serving cwt bank vs reference")
+    lib = G.L()
+    n, fs = 1024, 16000.0
+    fmin = 16 * fs / n
+    t = _cwt_create(n, fs, 8, fmin, fmin * 2 ** (31.5 / 8))
+    try:
+        assert lib.faf_cwt_n_scales(t) == 32
+        tab, _ = _table(t, lib)
+        ref = golden["cwt1024v8_psi"]
+        assert rel_l2(tab[:, :ref.shape[1]], ref) < 1e-6
+        fr = np.zeros(32)
+        lib.faf_cwt_freqs(t, fr.ctypes.data_as(C.POINTER(C.c_double)), 32)
+        assert np.allclose(fr, golden["cwt1024v8_freqs"], rtol=1e-12)
+        rep = lib.faf_cwt_bank_report(t).contents
+        g = golden["cwt1024v8_report"]
+        assert rep.n_scales == g[0] and rep.passed == g[1] and rep.k_lo == g[7]
+        assert abs(rep.frame_cond - g[6]) < 1e-9
+    finally:
+        lib.faf_destroy_transform(t)
+
+
+def test_single_frame_stft_matches_reference_golden(golden):
+    say("case: single-frame stft vs reference golden")
+    z = golden["stft1_1000_in"]
+    t = G.create("stft", 1000, layout="interleaved", win_length=800)
+    try:
+        y = G.c2c(t, z[None], inter=True)[0]
+        assert rel_l2(y, golden["stft1_1000_out"]) < tol(1000)
+    finally:
+        G.destroy(t)

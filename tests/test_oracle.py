@@ -242,6 +242,52 @@ def test_icwt_l1_oracle_against_reference(golden):
     assert rel_l2(O.icwt_l1_f32(w, w.shape[0] - 1, True, v, 2 * C_), ref) > 1e-2
 
 
+@pytest.mark.parametrize("n", [512, 4096])
+def test_icwt_dual_oracle_against_reference(golden, n):
+    """The oracle's FP64 dual-frame inverse against the compiled reference's
+    FP32 dual inverse (FAF_CWT_INV_DUAL) of the reference's own rows: the
+    default n=512 bank and the 64-scale bank at n=4096."""
+    if n == 512:
+        w, psi, ref = golden["cwt512_rows"], golden["cwt512_psi"], golden["cwt512_dual_out"]
+    else:
+        w, psi, ref = (golden["cwt4096_seg_rows"], golden["cwt4096_psi"],
+                       golden["cwt4096_dual_out"])
+    y = O.icwt_dual64(w, psi)
+    assert rel_l2(y, ref) < tol(n)
+    # the pieces matter: without the lowpass row, with an unregularised
+    # dual, or with a row order mistake the result moves off the reference
+    assert rel_l2(O.icwt_dual64(w[1:], psi[1:]), ref) > 1e-3
+    assert rel_l2(O.icwt_dual64(w[::-1].copy(), psi), ref) > 1e-2
+    # and it reconstructs the segment the rows came from (n=4096 input kept)
+    if n == 4096:
+        assert rel_l2(y, golden["cwt4096_seg_in"]) < 1e-4
+
+
+def test_single_frame_stft_reference_is_windowed_fft(golden):
+    """The reference's single-frame STFT (VM path, interleaved, n=1000,
+    win 800) is the FFT of the input times a symmetric Hann window of 800
+    samples, zero past it (src/faf_core.c:994-1032, 1369-1383)."""
+    z = golden["stft1_1000_in"]
+    n, win = z.size, 800
+    w = np.zeros(n)
+    w[:win] = O.hann32(win)
+    truth = O.fft64(z.astype(np.complex128) * w)[0]
+    assert rel_l2(golden["stft1_1000_out"], truth) < tol(n)
+    assert rel_l2(golden["stft1_1000_out"], O.fft64(z.astype(np.complex128))[0]) > 0.1
+
+
+def test_serving_bank_golden_invariants(golden):
+    """The serving bank (n=1024, 8 voices, 32 scales from 16 fs/n) written by
+    the reference: 32 scales at ratio 2^(1/8), frame condition < 1.2 and
+    passed (reference tests/test_cwt_lp.cpp:67-90, 193-215)."""
+    f = golden["cwt1024v8_freqs"]
+    rep = golden["cwt1024v8_report"]
+    assert f.size == 32 and golden["cwt1024v8_psi"].shape == (33, 513)
+    assert np.allclose(f[:-1] / f[1:], 2 ** (1 / 8), rtol=1e-9) or \
+        np.allclose(f[1:] / f[:-1], 2 ** (1 / 8), rtol=1e-9)
+    assert rep[1] == 1 and rep[6] < 1.2
+
+
 def test_icwt_l1_oracle_sine_round_trip(golden):
     """Reference tests/test_cwt.cpp:172-204: a sine at the bank's middle
     frequency through the forward CWT and the L1 inverse comes back within
