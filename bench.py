@@ -449,9 +449,27 @@ def run_extras(lib, fa, torch, np, timed, extra, only, err):
         xr = torch.rand(tot, device="cuda") * 2 - 1
         xi = torch.rand(tot, device="cuda") * 2 - 1
         yr, yi = torch.empty_like(xr), torch.empty_like(xi)
-        ms = timed(lambda: fa.check(lib.faf_ragged_execute(r, yr.data_ptr(), yi.data_ptr(), xr.data_ptr(), xi.data_ptr())), 50, 20)
+        run = lambda: fa.check(lib.faf_ragged_execute(r, yr.data_ptr(), yi.data_ptr(), xr.data_ptr(), xi.data_ptr()))
+        ms0 = timed(run, 50, 20)                 # the planner's default choices
+        # the ragged autotuner picks, per length, compile-time or runtime
+        # passes, the radix order and the signals per unit (outside the
+        # timed region); the batch is then timed with its choices
+        t0 = time.perf_counter()
+        fa.check(lib.faf_ragged_autotune(r) < 0, "ragged autotune")
+        extra["c3_autotune_s"] = round(time.perf_counter() - t0, 2)
+        ms = timed(run, 50, 20)
+        extra["c3_ragged_default_ms"] = round(ms0, 3)
         extra["c3_ragged_20000_ms"] = round(ms, 3)
         extra["c3_ragged_gbs"] = round(16 * tot / (ms * 1e-3) / 1e9, 1)
+        kinds = {0: "runtime", 1: "rader", 2: "bluestein", 3: "compile-time", 4: "rader-ct"}
+        chosen = {}
+        for L_ in lens_set:
+            k_, spb_, no_ = C.c_int(), C.c_int(), C.c_int()
+            order_ = (C.c_int * 16)()
+            if lib.faf_ragged_choice(r, L_, C.byref(k_), C.byref(spb_), order_, C.byref(no_)) == 0:
+                chosen[str(L_)] = {"kind": kinds.get(k_.value, k_.value), "spb": spb_.value,
+                                   "order": list(order_[: no_.value])}
+        extra["c3_tuned_choices"] = chosen
         # every 50th signal and the first of each length, against FP64
         off = np.concatenate([[0], np.cumsum(lens)])
         pick = set(range(0, len(lens), 50))

@@ -35,6 +35,7 @@ LAYOUT = {"default": 0, "split": 1, "hermitian": 2, "real": 3, "interleaved": 4}
 NORM = {"default": 0, "none": 1, "ortho": 2, "forward": 3}
 DIR = {"forward": 0, "inverse": 1}
 FLAG_BLUESTEIN = 1 << 12
+FLAG_MEASURE = 1 << 13
 CWT_FLAG_INCLUDE_LOWPASS = 0x1
 CWT_FLAG_ALLOW_UNTILED = 0x2
 CWT_FLAG_VALIDATE_STRICT = 0x20
@@ -173,6 +174,11 @@ _SIGS = {
     "faf_ragged_total": (c_size_t, [c_void_p]),
     "faf_ragged_execute": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "faf_ragged_destroy": (None, [c_void_p]),
+    "faf_ragged_autotune": (c_int, [c_void_p]),
+    "faf_ragged_choice": (c_int, [c_void_p, c_size_t, POINTER(c_int), POINTER(c_int),
+                                  POINTER(c_int), POINTER(c_int)]),
+    "faf_amd_plan_shape": (c_int, [PT, POINTER(c_int), POINTER(c_int), POINTER(c_int),
+                                   POINTER(c_int)]),
     "faf_stft_frames": (c_size_t, [PT, c_size_t]),
     "faf_execute_stft": (c_int, [PT, c_void_p, c_void_p, c_void_p, c_size_t, c_size_t,
                                  c_size_t]),

@@ -328,6 +328,21 @@ static int check_precision(faf_precision p, const char *what) {
     return -1;
 }
 
+/* FAF_FLAG_MEASURE (reference src/faf_core.c:539-565, called at :694):
+ * time the plan's launch shapes now when its key has no recorded shape (a
+ * recorded one was already applied while planning); the batch is 64 MB of
+ * complex signals */
+static int measure_at_create(faf_transform *t) {
+    fafh_plan *p = t->amd;
+    if (!(t->flags & FAF_FLAG_MEASURE) || !p) return 0;
+    if (p->kind != PK_C2C && p->kind != PK_R2C && p->kind != PK_C2R) return 0;
+    if (fafi_tune_find(p->kind, p->n, p->inverse, p->interleaved, NULL)) return 0;
+    size_t count = ((size_t)64 << 20) / (p->n * 8);
+    if (count < 16) count = 16;
+    if (count > 65536) count = 65536;
+    return fafi_autotune_plan(p, count) < 0 ? -1 : 0;
+}
+
 faf_transform *faf_create_fft(const faf_config *cfg) {
     /* planner order of reference src/faf_core.c:572-719 */
     if (!cfg) { fafi_err("faf_create_fft: cfg is NULL"); return NULL; }
@@ -367,6 +382,10 @@ faf_transform *faf_create_fft(const faf_config *cfg) {
         return NULL;
     }
     p->scale = norm_scale(n, c.norm, inv);
+    if (measure_at_create(t)) {
+        faf_destroy_transform(t);
+        return NULL;
+    }
     return t;
 }
 
@@ -415,6 +434,10 @@ faf_transform *faf_create_rfft(const faf_config *cfg) {
         return NULL;
     }
     p->scale = norm_scale(c.n, c.norm, inv);
+    if (measure_at_create(t)) {
+        faf_destroy_transform(t);
+        return NULL;
+    }
     return t;
 }
 

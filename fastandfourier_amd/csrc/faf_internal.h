@@ -41,9 +41,6 @@ int fafi_default_spb(const fafd_fft *P, size_t extra_per_signal);
 int fafi_fft64(size_t n, int inv, double *re, double *im);   /* host FP64 FFT, in place */
 int fafi_split_4step(size_t n, int *n1, int *n2);
 
-/* autotuned launch shape for (kind, n); 0 when unknown */
-int fafi_tuned_spb(int kind, size_t n);
-void fafi_tune_record(int kind, size_t n, int spb, double gbps);
 
 /* ---- the device plan behind faf_transform.amd ---- */
 typedef enum {
@@ -102,6 +99,25 @@ typedef struct fafh_plan {
     void *ws_ext;
     size_t ws_ext_bytes;
 } fafh_plan;
+
+/* ---- autotuned launch shapes (faf_tune.c), kept for the process and
+ * keyed by (plan kind, length, direction, interleaved layout) ---- */
+typedef struct fafi_shape {
+    int kind;
+    size_t n;
+    int inverse, interleaved;
+    int specialised;          /* the compile-time / single-pass kernel */
+    int norder;               /* otherwise: runtime radix order ... */
+    int order[FAFD_MAX_STAGES];
+    int spb;                  /* ... and signals per workgroup */
+    double ms;                /* measured median time of the tuning batch */
+} fafi_shape;
+int fafi_tune_find(int kind, size_t n, int inverse, int interleaved, fafi_shape *out);
+/* applies the recorded shape of p's key, if any (end of plan building) */
+int fafi_apply_tuned(fafh_plan *p);
+/* times p's candidate shapes on `count` signals, applies and records the
+ * fastest; returns its spb or -1 */
+int fafi_autotune_plan(fafh_plan *p, size_t count);
 
 fafh_plan *fafi_plan_new(void);
 void fafi_plan_free(fafh_plan *p);

@@ -571,9 +571,9 @@ int fafi_plan_c2c(fafh_plan *p, size_t n, int inv, int bluestein) {
         if (fafi_make_fft(n, inv, 1, NULL, 0, &p->a)) return -1;
         p->kind = PK_C2C;
         p->ct = fafi_ct_enabled() && fafk_ct_supports(0, (int)n);
-        int t = fafi_tuned_spb(PK_C2C, n);
-        p->spb = t > 0 ? t : fafi_default_spb(&p->a, 0);
-        return p->spb >= 1 ? 0 : -1;
+        p->spb = fafi_default_spb(&p->a, 0);
+        if (p->spb < 1) return -1;
+        return fafi_apply_tuned(p);
     }
     if (plan_4step(p, n, inv)) return -1;
     p->kind = PK_C2C_4S;
@@ -599,9 +599,9 @@ int fafi_plan_real(fafh_plan *p, size_t n, int inv) {
          * irfft 4096 (the rfft 4096 one-signal-per-workgroup kernel is faster
          * than the single pass) */
         p->rs = fafi_ct_enabled() && (nc == 8192 || (nc == 2048 && inv));
-        int t = fafi_tuned_spb(p->kind, n);
-        p->spb = t > 0 ? t : fafi_default_spb(&p->a, 0);
-        return p->spb >= 1 ? 0 : -1;
+        p->spb = fafi_default_spb(&p->a, 0);
+        if (p->spb < 1) return -1;
+        return fafi_apply_tuned(p);
     }
     if (plan_4step(p, nc, inv)) return -1;
     p->kind = inv ? PK_C2R_4S : PK_R2C_4S;
@@ -622,9 +622,9 @@ int fafi_plan_pipe(fafh_plan *p, size_t n) {
         return -1;
     p->kind = PK_PIPE;
     p->ct = fafi_ct_enabled() && fafk_ct_supports(3, (int)nc);
-    int t = fafi_tuned_spb(PK_PIPE, n);
-    p->spb = t > 0 ? t : fafi_default_spb(&p->a, 0);
-    return p->spb >= 1 ? 0 : -1;
+    p->spb = fafi_default_spb(&p->a, 0);
+    if (p->spb < 1) return -1;
+    return fafi_apply_tuned(p);
 }
 
 int fafi_plan_stft(fafh_plan *p, size_t win, size_t hop) {
@@ -650,37 +650,6 @@ int fafi_plan_stft(fafh_plan *p, size_t win, size_t hop) {
     p->hop = hop;
     p->kind = PK_STFT;
     p->ct = fafi_ct_enabled() && fafk_ct_supports(4, (int)nc);
-    int t = fafi_tuned_spb(PK_STFT, win);
-    p->spb = t > 0 ? t : fafi_default_spb(&p->a, 0);
+    p->spb = fafi_default_spb(&p->a, 0);
     return p->spb >= 1 ? 0 : -1;
-}
-
-/* ------------------------------------------------------------------ */
-/* autotune record (process-wide)                                      */
-/* ------------------------------------------------------------------ */
-
-typedef struct tune_ent { int kind; size_t n; int spb; double gbps; } tune_ent;
-static tune_ent g_tune[256];
-static int g_ntune;
-static pthread_mutex_t g_tune_mu = PTHREAD_MUTEX_INITIALIZER;
-
-int fafi_tuned_spb(int kind, size_t n) {
-    int r = 0;
-    pthread_mutex_lock(&g_tune_mu);
-    for (int i = 0; i < g_ntune; ++i)
-        if (g_tune[i].kind == kind && g_tune[i].n == n) { r = g_tune[i].spb; break; }
-    pthread_mutex_unlock(&g_tune_mu);
-    return r;
-}
-
-void fafi_tune_record(int kind, size_t n, int spb, double gbps) {
-    pthread_mutex_lock(&g_tune_mu);
-    int i;
-    for (i = 0; i < g_ntune; ++i)
-        if (g_tune[i].kind == kind && g_tune[i].n == n) break;
-    if (i == g_ntune && g_ntune < 256) ++g_ntune;
-    if (i < 256) {
-        g_tune[i].kind = kind; g_tune[i].n = n; g_tune[i].spb = spb; g_tune[i].gbps = gbps;
-    }
-    pthread_mutex_unlock(&g_tune_mu);
 }

@@ -1,6 +1,6 @@
 /*
  * faf_ragged.c -- many C2C signals of uneven lengths in one persistent
- * launch (C11 host side).
+ * launch (C11 host side), and the autotuner of that launch.
  *
  * Signals are bucketed by length; each bucket has one device plan
  * (Stockham, Rader or Bluestein) in a device-resident plan table.  Work is
@@ -8,8 +8,17 @@
  * and the units are ordered largest-first; persistent workgroups pull units
  * from an atomic counter until the queue is empty, so lengths from 11 to
  * 8192 share one launch without wave divergence inside a unit.
+ *
+ * How each length runs -- compile-time passes or a runtime radix plan (and
+ * which radix order), and how many signals a unit carries -- is a per-length
+ * choice.  By default it is the planner's; faf_ragged_autotune times the
+ * candidates of every length of a batch on the device, keeps the fastest
+ * per length (and whether the whole launch is faster without the runtime
+ * paths compiled in), and records the winners process-wide so that later
+ * ragged plans containing those lengths use them.
  */
 #include "faf_internal.h"
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -18,8 +27,18 @@
 /* with compile-time paths only, two workgroups share a CU's 160 KB */
 #define RAGGED_LDS_CAP_CT (79 * 1024)
 
+/* how one length runs in the launch */
+typedef struct rg_choice {
+    int kind;                 /* FAFD_KIND_*, or -1: the planner's default */
+    int norder;               /* runtime Stockham: radix order (0 = default) */
+    int order[FAFD_MAX_STAGES];
+    int spb;                  /* signals per unit (0 = as many as fit) */
+} rg_choice;
+
 struct faf_ragged {
     size_t count, total;
+    size_t *lengths;          /* the caller's lengths, kept for the autotuner */
+    uint32_t flags;
     int nplans;
     fafh_plan **hp;           /* host plans own the device tables */
     void *d_plans, *d_units, *d_order, *d_off, *d_counter;
@@ -28,6 +47,32 @@ struct faf_ragged {
     int grid;
     size_t lds;
 };
+
+/* ---- per-length choices kept for the process (autotuner results) ---- */
+
+typedef struct rg_tuned { size_t len; rg_choice c; } rg_tuned;
+static rg_tuned g_rg_tuned[256];
+static int g_rg_ntuned;
+static pthread_mutex_t g_rg_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static int rg_tuned_find(size_t len, rg_choice *out) {
+    int hit = 0;
+    pthread_mutex_lock(&g_rg_mu);
+    for (int i = 0; i < g_rg_ntuned; ++i)
+        if (g_rg_tuned[i].len == len) { *out = g_rg_tuned[i].c; hit = 1; break; }
+    pthread_mutex_unlock(&g_rg_mu);
+    return hit;
+}
+
+static void rg_tuned_store(size_t len, const rg_choice *c) {
+    pthread_mutex_lock(&g_rg_mu);
+    int i;
+    for (i = 0; i < g_rg_ntuned; ++i)
+        if (g_rg_tuned[i].len == len) break;
+    if (i == g_rg_ntuned && g_rg_ntuned < 256) ++g_rg_ntuned;
+    if (i < 256) { g_rg_tuned[i].len = len; g_rg_tuned[i].c = *c; }
+    pthread_mutex_unlock(&g_rg_mu);
+}
 
 static int cmp_size(const void *a, const void *b) {
     size_t x = *(const size_t *)a, y = *(const size_t *)b;
@@ -48,26 +93,115 @@ static void *up(const void *h, size_t bytes) {
     return d;
 }
 
-void faf_ragged_destroy(faf_ragged *r) {
-    if (!r) return;
+static void ragged_free_device(faf_ragged *r) {
     for (int i = 0; i < r->nplans; ++i) fafi_plan_free(r->hp[i]);
     free(r->hp);
+    r->hp = NULL;
+    r->nplans = 0;
     fafk_free(r->d_plans); fafk_free(r->d_units); fafk_free(r->d_order);
     fafk_free(r->d_off); fafk_free(r->d_counter);
+    r->d_plans = r->d_units = r->d_order = r->d_off = r->d_counter = NULL;
+}
+
+void faf_ragged_destroy(faf_ragged *r) {
+    if (!r) return;
+    ragged_free_device(r);
+    free(r->lengths);
     free(r);
 }
 
 size_t faf_ragged_total(const faf_ragged *r) { return r ? r->total : 0; }
 
-faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flags) {
-    if (!lengths || count == 0 || count > 0x7fffffff) {
-        fafi_err("ragged: need 1..2^31-1 lengths");
-        return NULL;
+/* The device plan of one length under choice c (NULL or kind -1: the
+ * planner's default).  Returns 0 and fills d / *per (LDS float2 per signal),
+ * or -1 with an error.  A choice that the length cannot take (compile-time
+ * passes it has none of) fails, so the autotuner skips it. */
+static int plan_length(fafh_plan *p, size_t n, uint32_t flags, const rg_choice *c,
+                       fafd_rplan *d, size_t *per) {
+    if (fafi_plan_c2c(p, n, 0, (flags & FAF_FLAG_BLUESTEIN) != 0)) return -1;
+    memset(d, 0, sizeof *d);
+    d->len = (int32_t)n;
+    const int want = c ? c->kind : -1;
+    if (p->kind == PK_C2C) {
+        /* the planner's own radix order: the compile-time passes of the
+         * 5-smooth lengths follow it (a plan-level autotune may have changed
+         * p->a, so it is rebuilt here) */
+        fafd_fft def;
+        if (fafi_make_fft(n, 0, 1, NULL, 0, &def)) return -1;
+        const int ct_t = fafi_ct_enabled() && n <= 8192 ? fafk_rg_ct_threads((int)n) : 0;
+        const int pow2 = !(n & (n - 1));
+        const int ct_ok = ct_t > 0 && (pow2 || ct_t == def.threads);
+        int kind = want >= 0 ? want : (ct_ok ? FAFD_KIND_CT : FAFD_KIND_STOCKHAM);
+        if (kind == FAFD_KIND_CT) {
+            if (!ct_ok) { fafi_err("ragged: length %zu has no compile-time passes", n); return -1; }
+            d->kind = FAFD_KIND_CT;
+            d->fwd = def;
+            d->fwd.threads = ct_t;
+            *per = (size_t)def.plane;
+            if (fafk_rg_ct_slots((int)n) > 0) *per = (size_t)fafk_rg_ct_slots((int)n);
+        } else if (kind == FAFD_KIND_STOCKHAM) {
+            d->kind = FAFD_KIND_STOCKHAM;
+            if (c && c->norder > 0) {
+                if (fafi_make_fft(n, 0, 1, c->order, c->norder, &d->fwd)) return -1;
+            } else {
+                d->fwd = def;
+            }
+            *per = (size_t)d->fwd.plane;
+        } else {
+            fafi_err("ragged: length %zu cannot run as kind %d", n, kind);
+            return -1;
+        }
+    } else if (p->kind == PK_RADER) {
+        const int rt = fafi_ct_enabled() ? fafk_rg_rader_threads((int)n) : 0;
+        int kind = want >= 0 ? want : (rt > 0 ? FAFD_KIND_RADER_CT : FAFD_KIND_RADER);
+        if (kind == FAFD_KIND_RADER_CT && rt <= 0) {
+            fafi_err("ragged: prime %zu has no compile-time inner passes", n);
+            return -1;
+        }
+        if (kind != FAFD_KIND_RADER_CT && kind != FAFD_KIND_RADER) {
+            fafi_err("ragged: prime %zu cannot run as kind %d", n, kind);
+            return -1;
+        }
+        d->kind = kind;
+        d->fwd = p->a; d->inv = p->b; d->rader = p->rader;
+        if (kind == FAFD_KIND_RADER_CT) d->fwd.threads = rt;
+        *per = (size_t)p->a.plane + n + (n >> 4) + 1;
+    } else if (p->kind == PK_BLUE) {
+        if (want >= 0 && want != FAFD_KIND_BLUESTEIN) {
+            fafi_err("ragged: length %zu runs through Bluestein only", n);
+            return -1;
+        }
+        d->kind = FAFD_KIND_BLUESTEIN;
+        d->fwd = p->a; d->inv = p->b; d->blue = p->blue;
+        *per = (size_t)p->a.plane;
+    } else {
+        fafi_err("ragged: length %zu exceeds one workgroup", n);
+        return -1;
     }
-    if (fafi_create_ready()) return NULL;
-    faf_ragged *r = calloc(1, sizeof *r);
-    r->count = count;
-    /* distinct lengths */
+    d->threads = d->fwd.threads;
+    if (d->threads > RAGGED_BLOCK) {
+        fafi_err("ragged: length %zu needs %d threads", n, d->threads);
+        return -1;
+    }
+    return 0;
+}
+
+/* choice of length n: the caller's override table first, then the
+ * process-wide autotuned table, else the default */
+typedef struct { const size_t *len; const rg_choice *c; int n; } rg_overrides;
+
+static const rg_choice *choice_for(size_t n, const rg_overrides *ov, rg_choice *tmp) {
+    if (ov)
+        for (int i = 0; i < ov->n; ++i)
+            if (ov->len[i] == n) return &ov->c[i];
+    if (rg_tuned_find(n, tmp)) return tmp;
+    return NULL;
+}
+
+/* Builds the device tables of a ragged batch into r (r->lengths/count set). */
+static int ragged_build(faf_ragged *r, const rg_overrides *ov) {
+    const size_t count = r->count;
+    const size_t *lengths = r->lengths;
     size_t *uniq = malloc(count * sizeof(size_t));
     memcpy(uniq, lengths, count * sizeof(size_t));
     qsort(uniq, count, sizeof(size_t), cmp_size);
@@ -77,58 +211,36 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
     r->hp = calloc((size_t)nu, sizeof(fafh_plan *));
     fafd_rplan *tab = calloc((size_t)nu, sizeof(fafd_rplan));
     size_t *pers = calloc((size_t)nu, sizeof(size_t));
+    int *want_spb = calloc((size_t)nu, sizeof(int));
     int ok = 1;
     size_t lds_max = 0;
+    r->generic = 0;
     for (int k = 0; k < nu && ok; ++k) {
         fafh_plan *p = fafi_plan_new();
         r->hp[r->nplans++] = p;
-        if (fafi_plan_c2c(p, uniq[k], 0, (flags & FAF_FLAG_BLUESTEIN) != 0)) { ok = 0; break; }
-        fafd_rplan *d = &tab[k];
-        d->len = (int32_t)uniq[k];
-        size_t per;
-        if (p->kind == PK_C2C) {
-            /* lengths with compile-time passes in the ragged kernel; the
-             * 5-smooth ones share the runtime plan's thread count */
-            const size_t n = uniq[k];
-            const int ct_t = fafi_ct_enabled() && n <= 8192 ? fafk_rg_ct_threads((int)n) : 0;
-            const int pow2 = !(n & (n - 1));
-            const int ct = ct_t > 0 && (pow2 || ct_t == p->a.threads);
-            d->kind = ct ? FAFD_KIND_CT : FAFD_KIND_STOCKHAM;
-            d->fwd = p->a;
-            if (ct) d->fwd.threads = ct_t;
-            per = (size_t)p->a.plane;
-            if (ct && fafk_rg_ct_slots((int)n) > 0) per = (size_t)fafk_rg_ct_slots((int)n);
-        } else if (p->kind == PK_RADER) {
-            const int rt = fafi_ct_enabled() ? fafk_rg_rader_threads((int)uniq[k]) : 0;
-            d->kind = rt > 0 ? FAFD_KIND_RADER_CT : FAFD_KIND_RADER;
-            d->fwd = p->a; d->inv = p->b; d->rader = p->rader;
-            if (rt > 0) d->fwd.threads = rt;
-            per = (size_t)p->a.plane + uniq[k] + (uniq[k] >> 4) + 1;
-        } else if (p->kind == PK_BLUE) {
-            d->kind = FAFD_KIND_BLUESTEIN;
-            d->fwd = p->a; d->inv = p->b; d->blue = p->blue;
-            per = (size_t)p->a.plane;
-        } else {
-            fafi_err("ragged: length %zu exceeds one workgroup", uniq[k]);
-            ok = 0;
-            break;
+        rg_choice tmp;
+        const rg_choice *c = choice_for(uniq[k], ov, &tmp);
+        if (plan_length(p, uniq[k], r->flags, c, &tab[k], &pers[k])) {
+            /* a recorded choice this length cannot take falls back to the
+             * default; an override from the autotuner's trial must hold */
+            if (!c || (ov && c != &tmp)) { ok = 0; break; }
+            fafi_plan_free(p);
+            r->hp[r->nplans - 1] = p = fafi_plan_new();
+            c = NULL;
+            if (plan_length(p, uniq[k], r->flags, NULL, &tab[k], &pers[k])) { ok = 0; break; }
         }
-        d->threads = d->fwd.threads;
-        if (d->threads > RAGGED_BLOCK) {
-            fafi_err("ragged: length %zu needs %d threads", uniq[k], d->threads);
-            ok = 0;
-            break;
-        }
-        if (d->kind != FAFD_KIND_CT && d->kind != FAFD_KIND_RADER_CT) r->generic = 1;
-        pers[k] = per;
+        want_spb[k] = c ? c->spb : 0;
+        if (tab[k].kind != FAFD_KIND_CT && tab[k].kind != FAFD_KIND_RADER_CT) r->generic = 1;
     }
-    /* signals per unit: as many as the threads and the LDS cap allow */
+    /* signals per unit: as many as the threads and the LDS cap allow, or
+     * fewer when the choice asks */
     const size_t cap = r->generic ? RAGGED_LDS_CAP : RAGGED_LDS_CAP_CT;
     for (int k = 0; k < nu && ok; ++k) {
         fafd_rplan *d = &tab[k];
         const size_t per = pers[k];
         int spb = RAGGED_BLOCK / d->threads;
         while (spb > 1 && (size_t)spb * per * 8 > cap) --spb;
+        if (want_spb[k] > 0 && want_spb[k] < spb) spb = want_spb[k];
         if ((size_t)spb * per * 8 > FAFD_MAX_LDS_BYTES) {
             fafi_err("ragged: length %zu does not fit LDS", uniq[k]);
             ok = 0;
@@ -138,7 +250,8 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
         if ((size_t)spb * per * 8 > lds_max) lds_max = (size_t)spb * per * 8;
     }
     free(pers);
-    if (!ok) { free(uniq); free(tab); faf_ragged_destroy(r); return NULL; }
+    free(want_spb);
+    if (!ok) { free(uniq); free(tab); return -1; }
 
     /* order signals by plan, offsets as exclusive prefix sums */
     int32_t *order = malloc(count * sizeof(int32_t));
@@ -200,10 +313,50 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
     free(fill); free(uc); free(units);
     if (!r->d_plans || !r->d_units || !r->d_order || !r->d_off || !r->d_counter) {
         fafi_err("ragged: %s", fafk_last_error());
+        return -1;
+    }
+    return 0;
+}
+
+static faf_ragged *ragged_new(const size_t *lengths, size_t count, uint32_t flags,
+                              const rg_overrides *ov) {
+    faf_ragged *r = calloc(1, sizeof *r);
+    if (!r) { fafi_err("ragged: alloc failed"); return NULL; }
+    r->count = count;
+    r->flags = flags;
+    r->lengths = malloc(count * sizeof(size_t));
+    if (!r->lengths) { free(r); fafi_err("ragged: alloc failed"); return NULL; }
+    memcpy(r->lengths, lengths, count * sizeof(size_t));
+    if (ragged_build(r, ov)) { faf_ragged_destroy(r); return NULL; }
+    return r;
+}
+
+faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flags) {
+    if (!lengths || count == 0 || count > 0x7fffffff) {
+        fafi_err("ragged: need 1..2^31-1 lengths");
+        return NULL;
+    }
+    for (size_t i = 0; i < count; ++i)
+        if (lengths[i] < 2) { fafi_err("ragged: length %zu is not a transform", lengths[i]); return NULL; }
+    if (fafi_create_ready()) return NULL;
+    faf_ragged *r = ragged_new(lengths, count, flags, NULL);
+    /* the reference's MEASURE flag (src/faf_core.c:539-565, called at :694)
+     * asks for timing at create: tune the batch's lengths that have no
+     * recorded choice yet */
+    if (r && (flags & FAF_FLAG_MEASURE) && faf_ragged_autotune(r) < 0) {
         faf_ragged_destroy(r);
         return NULL;
     }
     return r;
+}
+
+static int launch(faf_ragged *r, float *out_re, float *out_im, const float *in_re,
+                  const float *in_im, void *stream) {
+    int rc = fafk_ragged(r->d_plans, r->d_units, r->nunits, r->d_order, r->d_off, in_re, in_im,
+                         out_re, out_im, r->d_counter, r->grid, RAGGED_BLOCK, r->lds,
+                         r->generic, stream);
+    if (rc) fafi_err("ragged: %s", fafk_rg_last_error());
+    return rc;
 }
 
 int faf_ragged_execute(faf_ragged *r, float *out_re, float *out_im, const float *in_re,
@@ -220,10 +373,243 @@ int faf_ragged_execute(faf_ragged *r, float *out_re, float *out_im, const float 
         fafi_err("ragged: buffers must be device memory");
         return -1;
     }
-    int rc = fafk_ragged(r->d_plans, r->d_units, r->nunits, r->d_order, r->d_off, in_re, in_im,
-                         out_re, out_im, r->d_counter, r->grid, RAGGED_BLOCK, r->lds, r->generic,
-                         stream);
+    int rc = launch(r, out_re, out_im, in_re, in_im, stream);
     if (!rc && fafi_sync_mode()) rc = fafk_sync(stream);
-    if (rc) fafi_err("ragged: %s", fafk_rg_last_error());
     return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* autotuner of the ragged launch                                       */
+/* ------------------------------------------------------------------ */
+
+#define RG_MAX_CANDS 24
+
+/* median launch time (ms) of r over 5 timed launches after 2 warm ones */
+static float time_ragged(faf_ragged *r, float *buf, void *stream) {
+    float *ir = buf, *ii = buf + r->total, *orr = buf + 2 * r->total, *oi = buf + 3 * r->total;
+    for (int w = 0; w < 2; ++w)
+        if (launch(r, orr, oi, ir, ii, stream)) return -1.0f;
+    float t[5];
+    for (int k = 0; k < 5; ++k) {
+        void *e0 = fafk_event_create(), *e1 = fafk_event_create();
+        fafk_event_record(e0, stream);
+        int rc = launch(r, orr, oi, ir, ii, stream);
+        fafk_event_record(e1, stream);
+        t[k] = rc ? -1.0f : (fafk_event_sync(e1), fafk_event_ms(e0, e1));
+        fafk_event_destroy(e0);
+        fafk_event_destroy(e1);
+        if (t[k] < 0.0f) return -1.0f;
+    }
+    for (int i = 1; i < 5; ++i)         /* insertion sort, then the median */
+        for (int j = i; j > 0 && t[j] < t[j - 1]; --j) { float s = t[j]; t[j] = t[j - 1]; t[j - 1] = s; }
+    return t[2];
+}
+
+/* candidate choices of one length */
+static int candidates(size_t n, uint32_t flags, rg_choice *out) {
+    int nc = 0;
+    fafh_plan *p = fafi_plan_new();
+    if (!p || fafi_plan_c2c(p, n, 0, (flags & FAF_FLAG_BLUESTEIN) != 0)) {
+        fafi_plan_free(p);
+        return 0;
+    }
+    const int kind = p->kind;
+    fafd_fft def = p->a;
+    fafi_plan_free(p);
+    const int spbs[3] = {0, -2, -4};      /* all that fit, half of that, a quarter */
+    int kinds[2], nk = 0;
+    if (kind == PK_C2C) { kinds[nk++] = FAFD_KIND_CT; kinds[nk++] = FAFD_KIND_STOCKHAM; }
+    else if (kind == PK_RADER) { kinds[nk++] = FAFD_KIND_RADER_CT; kinds[nk++] = FAFD_KIND_RADER; }
+    else kinds[nk++] = FAFD_KIND_BLUESTEIN;
+    int orders[8][FAFD_MAX_STAGES], lens[8], no = 0;
+    if (kind == PK_C2C) {
+        int rad[FAFD_MAX_STAGES], k;
+        if (fafi_make_fft(n, 0, 1, NULL, 0, &def) == 0) {
+            k = def.nstages;
+            for (int s = 0; s < k; ++s) rad[s] = def.st[s].radix;
+            for (int rot = 0; rot < k && no < 7; ++rot) {      /* rotations of the default */
+                for (int s = 0; s < k; ++s) orders[no][s] = rad[(s + rot) % k];
+                lens[no++] = k;
+            }
+            int m = 0, split[FAFD_MAX_STAGES];                  /* every 16 as 4 x 4 */
+            for (int s = 0; s < k && m < FAFD_MAX_STAGES - 1; ++s) {
+                if (rad[s] == 16) { split[m++] = 4; split[m++] = 4; }
+                else split[m++] = rad[s];
+            }
+            if (m != k && m <= FAFD_MAX_STAGES) {
+                memcpy(orders[no], split, (size_t)m * sizeof(int));
+                lens[no++] = m;
+            }
+        }
+    }
+    for (int a = 0; a < nk; ++a) {
+        const int nord = kinds[a] == FAFD_KIND_STOCKHAM ? no : 1;
+        for (int o = 0; o < nord; ++o)
+            for (int s = 0; s < 3 && nc < RG_MAX_CANDS; ++s) {
+                rg_choice c;
+                memset(&c, 0, sizeof c);
+                c.kind = kinds[a];
+                if (kinds[a] == FAFD_KIND_STOCKHAM) {
+                    c.norder = lens[o];
+                    memcpy(c.order, orders[o], (size_t)lens[o] * sizeof(int));
+                }
+                c.spb = spbs[s];          /* negative: a fraction, resolved below */
+                out[nc++] = c;
+            }
+    }
+    return nc;
+}
+
+/* the most signals of length n a unit can take under choice c */
+static int max_spb(size_t n, uint32_t flags, const rg_choice *c, int generic) {
+    fafh_plan *p = fafi_plan_new();
+    fafd_rplan d;
+    size_t per = 0;
+    int rc = p ? plan_length(p, n, flags, c, &d, &per) : -1;
+    fafi_plan_free(p);
+    if (rc) return 0;
+    const size_t cap = generic ? RAGGED_LDS_CAP : RAGGED_LDS_CAP_CT;
+    int spb = RAGGED_BLOCK / d.threads;
+    while (spb > 1 && (size_t)spb * per * 8 > cap) --spb;
+    return spb;
+}
+
+int faf_ragged_autotune(faf_ragged *r) {
+    if (!r) { fafi_err("ragged autotune: null plan"); return -1; }
+    if (fafi_create_ready()) return -1;
+    void *stream = fafi_stream();
+    /* scratch for inputs and outputs of the largest trial: the whole batch
+     * (zeros: no kernel path depends on the data) */
+    const size_t bytes = 4 * r->total * sizeof(float);
+    float *buf = fafk_malloc(bytes);
+    if (!buf) { fafi_err("ragged autotune: %s", fafk_last_error()); return -1; }
+    float *zero = calloc(1, bytes);
+    if (!zero || fafk_h2d(buf, zero, bytes, stream) || fafk_sync(stream)) {
+        free(zero);
+        fafk_free(buf);
+        fafi_err("ragged autotune: %s", fafk_last_error());
+        return -1;
+    }
+    free(zero);
+
+    /* distinct lengths and how many signals each has */
+    size_t *uniq = malloc(r->count * sizeof(size_t));
+    memcpy(uniq, r->lengths, r->count * sizeof(size_t));
+    qsort(uniq, r->count, sizeof(size_t), cmp_size);
+    int nu = 0;
+    size_t *cnt = calloc(r->count, sizeof(size_t));
+    for (size_t i = 0; i < r->count; ++i) {
+        if (i == 0 || uniq[i] != uniq[i - 1]) uniq[nu++] = uniq[i];
+        cnt[nu - 1]++;
+    }
+    rg_choice *best = calloc((size_t)nu, sizeof(rg_choice));
+    rg_choice *best_ct = calloc((size_t)nu, sizeof(rg_choice));   /* best without runtime paths */
+    int *has_ct = calloc((size_t)nu, sizeof(int));
+    int tuned = 0, rc = 0;
+
+    for (int k = 0; k < nu && !rc; ++k) {
+        rg_choice known;
+        if (rg_tuned_find(uniq[k], &known)) {      /* already measured in this process */
+            best[k] = known;
+            best_ct[k] = known;
+            has_ct[k] = known.kind == FAFD_KIND_CT || known.kind == FAFD_KIND_RADER_CT;
+            continue;
+        }
+        /* a batch of this length alone: as many signals as the caller's
+         * batch holds (at least 64, so a unit count is realistic) */
+        size_t m = cnt[k] < 64 ? 64 : cnt[k];
+        while (m * uniq[k] > r->total && m > 1) --m;
+        size_t *one = malloc(m * sizeof(size_t));
+        for (size_t i = 0; i < m; ++i) one[i] = uniq[k];
+        rg_choice cands[RG_MAX_CANDS];
+        int nc = candidates(uniq[k], r->flags, cands);
+        float tb = 1e30f, tct = 1e30f;
+        for (int ci = 0; ci < nc; ++ci) {
+            rg_choice c = cands[ci];
+            const int gen = c.kind != FAFD_KIND_CT && c.kind != FAFD_KIND_RADER_CT;
+            if (c.spb < 0) {
+                const int mx = max_spb(uniq[k], r->flags, &c, gen);
+                if (mx <= 0) continue;
+                const int s = mx / -c.spb;
+                if (s < 1 || (c.spb != 0 && s == mx)) continue;
+                c.spb = s;
+            }
+            rg_overrides ov = {&uniq[k], &c, 1};
+            faf_ragged *trial = ragged_new(one, m, r->flags, &ov);
+            if (!trial) continue;            /* this length cannot take that choice */
+            float ms = time_ragged(trial, buf, stream);
+            const int lean = !trial->generic;
+            faf_ragged_destroy(trial);
+            if (ms <= 0.0f) continue;
+            if (ms < tb) { tb = ms; best[k] = c; }
+            if (lean && ms < tct) { tct = ms; best_ct[k] = c; has_ct[k] = 1; }
+        }
+        free(one);
+        if (tb >= 1e30f) { fafi_err("ragged autotune: no candidate of length %zu ran", uniq[k]); rc = -1; }
+        ++tuned;
+    }
+    if (!rc) {
+        fafi_err("%s", "");     /* candidates a length cannot take left messages */
+        /* whole-batch decision: the per-length winners (which may need the
+         * runtime paths, and with them the larger kernel) against the best
+         * compile-time choice of every length that has one */
+        rg_overrides ov_best = {uniq, best, nu};
+        rg_choice *lean = malloc((size_t)nu * sizeof(rg_choice));
+        for (int k = 0; k < nu; ++k) lean[k] = has_ct[k] ? best_ct[k] : best[k];
+        rg_overrides ov_lean = {uniq, lean, nu};
+        faf_ragged *a = ragged_new(r->lengths, r->count, r->flags, &ov_best);
+        faf_ragged *b = ragged_new(r->lengths, r->count, r->flags, &ov_lean);
+        float ta = a ? time_ragged(a, buf, stream) : -1.0f;
+        float tb = b ? time_ragged(b, buf, stream) : -1.0f;
+        const rg_choice *win = NULL;
+        if (ta > 0.0f && (tb <= 0.0f || ta <= tb)) win = best;
+        else if (tb > 0.0f) win = lean;
+        faf_ragged_destroy(a);
+        faf_ragged_destroy(b);
+        if (!win) {
+            fafi_err("ragged autotune: the batch did not run");
+            rc = -1;
+        } else {
+            for (int k = 0; k < nu; ++k) rg_tuned_store(uniq[k], &win[k]);
+            /* rebuild this plan with the winners */
+            ragged_free_device(r);
+            rg_overrides ov = {uniq, win, nu};
+            if (ragged_build(r, &ov)) rc = -1;
+        }
+        free(lean);
+    }
+    fafk_free(buf);
+    free(uniq); free(cnt); free(best); free(best_ct); free(has_ct);
+    return rc ? -1 : tuned;
+}
+
+int faf_ragged_choice(const faf_ragged *r, size_t len, int *kind, int *spb, int *order,
+                      int *norder) {
+    if (!r) return -1;
+    for (size_t i = 0; i < r->count; ++i) {
+        if (r->lengths[i] != len) continue;
+        /* the plan table is on the device; read the entry of this length */
+        int nu = r->nplans;
+        fafd_rplan *tab = malloc((size_t)nu * sizeof(fafd_rplan));
+        if (!tab || fafk_d2h(tab, r->d_plans, (size_t)nu * sizeof(fafd_rplan), NULL) ||
+            fafk_sync(NULL)) {
+            free(tab);
+            fafi_err("ragged: %s", fafk_last_error());
+            return -1;
+        }
+        int rc = -1;
+        for (int k = 0; k < nu; ++k) {
+            if ((size_t)tab[k].len != len) continue;
+            if (kind) *kind = tab[k].kind;
+            if (spb) *spb = tab[k].spb;
+            if (norder) *norder = tab[k].fwd.nstages;
+            if (order)
+                for (int s = 0; s < tab[k].fwd.nstages; ++s) order[s] = tab[k].fwd.st[s].radix;
+            rc = 0;
+        }
+        free(tab);
+        return rc;
+    }
+    fafi_err("ragged: length %zu is not in this batch", len);
+    return -1;
 }

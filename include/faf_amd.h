@@ -85,6 +85,23 @@ size_t faf_ragged_total(const faf_ragged *r);
 int faf_ragged_execute(faf_ragged *r, float *out_re, float *out_im, const float *in_re,
                        const float *in_im);
 void faf_ragged_destroy(faf_ragged *r);
+/* Autotunes the launch for this batch's lengths: for each length it times,
+ * on the device and on a batch of that length alone, every way the length
+ * can run (compile-time passes, or runtime radix passes in each candidate
+ * radix order) with several signals-per-unit counts, median of 5 launches
+ * each; then it times the whole batch with the per-length winners against
+ * the best compile-time choices (the launch without runtime paths uses a
+ * leaner kernel) and keeps the faster.  The winners are recorded for the
+ * process, so later faf_ragged_create calls use them for those lengths;
+ * faf_ragged_create with FAF_FLAG_MEASURE in flags runs this at create.
+ * Rebuilds r in place.  Returns the number of lengths timed (0 when every
+ * length already had a recorded choice), or -1. */
+int faf_ragged_autotune(faf_ragged *r);
+/* How length `len` runs in r: kind (0 runtime radix passes, 1 Rader, 2
+ * Bluestein, 3 compile-time passes, 4 Rader with compile-time inner
+ * passes), signals per unit, and the radix order of its (inner) FFT. */
+int faf_ragged_choice(const faf_ragged *r, size_t len, int *kind, int *spb, int *order,
+                      int *norder);
 
 /* ---- framed real STFT ----
  * faf_create_stft with cfg.layout == FAF_LAYOUT_REAL (an error in the
@@ -112,9 +129,19 @@ faf_transform *faf_create_pipeline(const faf_config *cfg, int nops, const int *o
 
 /* ---- autotuner ----
  * Times the candidate launch shapes of the plan on `count` synthetic
- * signals and keeps the fastest (cached per kind and length for the
- * process).  Returns the chosen signals-per-workgroup, or -1. */
+ * signals -- the specialised kernel where the length has one, and the
+ * runtime radix kernels in each candidate radix order with 1..64 signals
+ * per workgroup, each the median of 5 runs -- applies the fastest to the
+ * plan and records it for the process under the plan's key (kind, length,
+ * direction, interleaved layout): plans of that key created later start
+ * from it.  FAF_FLAG_MEASURE in cfg.flags runs this at create for FFT and
+ * RFFT plans whose key has no recorded shape.  Returns the chosen
+ * signals-per-workgroup, or -1. */
 int faf_amd_autotune(faf_transform *t, size_t count);
+/* The plan's launch shape: specialised kernel or not, the radix order of
+ * its (inner) FFT and signals per workgroup. */
+int faf_amd_plan_shape(const faf_transform *t, int *specialised, int *order, int *norder,
+                       int *spb);
 
 /* ---- kernel selection (thread-local, read at create time) ----
  * Power-of-two lengths 256..16384 (C2C) and 512..32768 (R2C/C2R, fused
