@@ -45,7 +45,8 @@ int fafi_split_4step(size_t n, int *n1, int *n2);
 /* ---- the device plan behind faf_transform.amd ---- */
 typedef enum {
     PK_NONE = 0, PK_C2C, PK_C2C_4S, PK_RADER, PK_BLUE, PK_R2C, PK_C2R, PK_R2C_4S, PK_C2R_4S,
-    PK_PIPE, PK_STFT, PK_CWT, PK_ICWT
+    PK_PIPE, PK_STFT, PK_CWT, PK_ICWT,
+    PK_RADER_L, PK_BLUE_L      /* Rader / Bluestein with inner FFTs beyond one workgroup */
 } fafh_kind;
 
 typedef struct fafh_plan {
@@ -94,6 +95,9 @@ typedef struct fafh_plan {
      * then a batched inverse over the rows (C2C for analytic rows, C2R for
      * real rows), each a plan of its own */
     struct fafh_plan *sub_r2c, *sub_c2c, *sub_c2r;
+    /* Rader / Bluestein beyond one workgroup: the inner forward and
+     * (unnormalised) inverse C2C of length p-1 or M */
+    struct fafh_plan *inner_f, *inner_i;
     /* caller-lent device workspace, used instead of ws while set (the
      * serving loop lends each request its own so streams never share) */
     void *ws_ext;

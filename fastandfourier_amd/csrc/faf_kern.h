@@ -103,6 +103,19 @@ int fafk_cwt(const fafd_fft *Ph, const fafd_fft *Pn, int real_out, const void *p
 int fafk_icwt(const fafd_fft *Pf, const fafd_fft *Pi, const void *post, const void *pre,
               const float *dual, long dual_stride, int rows, const float *w, long in_dist,
               float *y, long out_dist, long count, float scale, void *stream);
+/* Rader / Bluestein beyond one workgroup: element-wise steps around the
+ * inner FFTs (split planes, dists in elements) */
+int fafk_big_gather(const float *xr, const float *xi, long xdist, const int32_t *rin, long m,
+                    float *ar, float *ai, long adist, long count, void *stream);
+int fafk_big_chirp_in(const float *xr, const float *xi, long xdist, const void *chirp, long n,
+                      long M, float *ar, float *ai, long adist, long count, void *stream);
+int fafk_big_mul(float *ar, float *ai, long adist, const void *B, long m, void *save0,
+                 long count, void *stream);
+int fafk_big_rader_out(const float *ar, const float *ai, long adist, const int32_t *rout,
+                       long m, const float *xr, const float *xi, long xdist, const void *save0,
+                       float *yr, float *yi, long ydist, float scale, long count, void *stream);
+int fafk_big_chirp_out(const float *ar, const float *ai, long adist, const void *chirp, long n,
+                       float *yr, float *yi, long ydist, float scale, long count, void *stream);
 /* long-segment dual inverse CWT: acc = sum_r Y_r * dual_r over interleaved
  * row spectra Y [count][rows][nb]; acc interleaved [count][nb] */
 int fafk_icwt_mulacc(const float *Y, long nb, int rows, const float *dual, long stride,

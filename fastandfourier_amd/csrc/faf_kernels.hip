@@ -530,6 +530,161 @@ extern "C" int fafk_icwt_l1(const float *w, long in_dist, float *y, long out_dis
     return 0;
 }
 
+/* ---- Rader and Bluestein beyond one workgroup: the element-wise steps
+ * around the inner FFTs, which run through HBM.  Split planes; signal s of
+ * a plane starts at s * dist; one thread per element, bps blocks of 256
+ * threads per signal. ---- */
+
+/* Rader gather: a[j] = x[rin[j]] (rin[j] = g^j mod p), j < m = p - 1 */
+__global__ __launch_bounds__(256) void k_big_gather(const float *__restrict__ xr,
+                                                    const float *__restrict__ xi, long xdist,
+                                                    const int32_t *__restrict__ rin, long m,
+                                                    float *__restrict__ ar, float *__restrict__ ai,
+                                                    long adist, long bps) {
+    const long s = blockIdx.x / bps;
+    const long j = (blockIdx.x - s * bps) * 256 + threadIdx.x;
+    if (j >= m) return;
+    const long src = s * xdist + rin[j];
+    ar[s * adist + j] = xr[src];
+    ai[s * adist + j] = xi[src];
+}
+
+/* Bluestein chirp in: a[k] = x[k] chirp[k] for k < n, zero for n <= k < M */
+__global__ __launch_bounds__(256) void k_big_chirp_in(const float *__restrict__ xr,
+                                                      const float *__restrict__ xi, long xdist,
+                                                      const float2 *__restrict__ chirp, long n,
+                                                      long M, float *__restrict__ ar,
+                                                      float *__restrict__ ai, long adist,
+                                                      long bps) {
+    const long s = blockIdx.x / bps;
+    const long k = (blockIdx.x - s * bps) * 256 + threadIdx.x;
+    if (k >= M) return;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (k < n) v = cmul(make_float2(xr[s * xdist + k], xi[s * xdist + k]), chirp[k]);
+    ar[s * adist + k] = v.x;
+    ai[s * adist + k] = v.y;
+}
+
+/* spectrum times the precomputed kernel: A[j] *= B[j]; save0, when given,
+ * receives each signal's A[0] before the product (Rader's DC term) */
+__global__ __launch_bounds__(256) void k_big_mul(float *__restrict__ ar, float *__restrict__ ai,
+                                                 long adist, const float2 *__restrict__ B,
+                                                 long m, float2 *__restrict__ save0, long bps) {
+    const long s = blockIdx.x / bps;
+    const long j = (blockIdx.x - s * bps) * 256 + threadIdx.x;
+    if (j >= m) return;
+    const float2 a = make_float2(ar[s * adist + j], ai[s * adist + j]);
+    if (save0 && j == 0) save0[s] = a;
+    const float2 v = cmul(a, B[j]);
+    ar[s * adist + j] = v.x;
+    ai[s * adist + j] = v.y;
+}
+
+/* Rader scatter: y[rout[j]] = scale (x0 + a[j]); y[0] = scale (x0 + A0) */
+__global__ __launch_bounds__(256) void k_big_rader_out(const float *__restrict__ ar,
+                                                       const float *__restrict__ ai, long adist,
+                                                       const int32_t *__restrict__ rout, long m,
+                                                       const float *__restrict__ xr,
+                                                       const float *__restrict__ xi, long xdist,
+                                                       const float2 *__restrict__ save0,
+                                                       float *__restrict__ yr,
+                                                       float *__restrict__ yi, long ydist,
+                                                       float scale, long bps) {
+    const long s = blockIdx.x / bps;
+    const long j = (blockIdx.x - s * bps) * 256 + threadIdx.x;
+    if (j >= m) return;
+    const float2 x0 = make_float2(xr[s * xdist], xi[s * xdist]);
+    const float2 v = cadd(x0, make_float2(ar[s * adist + j], ai[s * adist + j]));
+    const long d = s * ydist + rout[j];
+    yr[d] = v.x * scale;
+    yi[d] = v.y * scale;
+    if (j == 0) {
+        const float2 dc = cadd(x0, save0[s]);
+        yr[s * ydist] = dc.x * scale;
+        yi[s * ydist] = dc.y * scale;
+    }
+}
+
+/* Bluestein chirp out: y[k] = scale a[k] chirp[k], k < n */
+__global__ __launch_bounds__(256) void k_big_chirp_out(const float *__restrict__ ar,
+                                                       const float *__restrict__ ai, long adist,
+                                                       const float2 *__restrict__ chirp, long n,
+                                                       float *__restrict__ yr,
+                                                       float *__restrict__ yi, long ydist,
+                                                       float scale, long bps) {
+    const long s = blockIdx.x / bps;
+    const long k = (blockIdx.x - s * bps) * 256 + threadIdx.x;
+    if (k >= n) return;
+    const float2 v = cmul(make_float2(ar[s * adist + k], ai[s * adist + k]), chirp[k]);
+    yr[s * ydist + k] = v.x * scale;
+    yi[s * ydist + k] = v.y * scale;
+}
+
+static int big_grid(long len, long count, long *bps, const char *what) {
+    *bps = (len + 255) / 256;
+    if (len <= 0 || count <= 0 || *bps * count > 0x7fffffffL) {
+        snprintf(g_err, sizeof g_err, "%s: unsupported shape len=%ld count=%ld", what, len, count);
+        return -1;
+    }
+    return 0;
+}
+
+extern "C" int fafk_big_gather(const float *xr, const float *xi, long xdist, const int32_t *rin,
+                               long m, float *ar, float *ai, long adist, long count,
+                               void *stream) {
+    long bps;
+    if (big_grid(m, count, &bps, "k_big_gather")) return -1;
+    hipLaunchKernelGGL(k_big_gather, dim3((unsigned)(bps * count)), dim3(256), 0, S(stream), xr,
+                       xi, xdist, rin, m, ar, ai, adist, bps);
+    LAUNCH_CHECK("k_big_gather");
+    return 0;
+}
+
+extern "C" int fafk_big_chirp_in(const float *xr, const float *xi, long xdist,
+                                 const void *chirp, long n, long M, float *ar, float *ai,
+                                 long adist, long count, void *stream) {
+    long bps;
+    if (big_grid(M, count, &bps, "k_big_chirp_in")) return -1;
+    hipLaunchKernelGGL(k_big_chirp_in, dim3((unsigned)(bps * count)), dim3(256), 0, S(stream),
+                       xr, xi, xdist, (const float2 *)chirp, n, M, ar, ai, adist, bps);
+    LAUNCH_CHECK("k_big_chirp_in");
+    return 0;
+}
+
+extern "C" int fafk_big_mul(float *ar, float *ai, long adist, const void *B, long m,
+                            void *save0, long count, void *stream) {
+    long bps;
+    if (big_grid(m, count, &bps, "k_big_mul")) return -1;
+    hipLaunchKernelGGL(k_big_mul, dim3((unsigned)(bps * count)), dim3(256), 0, S(stream), ar, ai,
+                       adist, (const float2 *)B, m, (float2 *)save0, bps);
+    LAUNCH_CHECK("k_big_mul");
+    return 0;
+}
+
+extern "C" int fafk_big_rader_out(const float *ar, const float *ai, long adist,
+                                  const int32_t *rout, long m, const float *xr, const float *xi,
+                                  long xdist, const void *save0, float *yr, float *yi,
+                                  long ydist, float scale, long count, void *stream) {
+    long bps;
+    if (big_grid(m, count, &bps, "k_big_rader_out")) return -1;
+    hipLaunchKernelGGL(k_big_rader_out, dim3((unsigned)(bps * count)), dim3(256), 0, S(stream),
+                       ar, ai, adist, rout, m, xr, xi, xdist, (const float2 *)save0, yr, yi,
+                       ydist, scale, bps);
+    LAUNCH_CHECK("k_big_rader_out");
+    return 0;
+}
+
+extern "C" int fafk_big_chirp_out(const float *ar, const float *ai, long adist,
+                                  const void *chirp, long n, float *yr, float *yi, long ydist,
+                                  float scale, long count, void *stream) {
+    long bps;
+    if (big_grid(n, count, &bps, "k_big_chirp_out")) return -1;
+    hipLaunchKernelGGL(k_big_chirp_out, dim3((unsigned)(bps * count)), dim3(256), 0, S(stream),
+                       ar, ai, adist, (const float2 *)chirp, n, yr, yi, ydist, scale, bps);
+    LAUNCH_CHECK("k_big_chirp_out");
+    return 0;
+}
+
 /* Dual-frame accumulation of the long-segment inverse CWT (reference
  * src/faf_cwt.c:958-977): acc[k] = sum over rows r, in row order, of
  * Y_r[k] * dual_r[k], with the imaginary parts of DC and Nyquist cleared.

@@ -35,6 +35,9 @@ static int krerr(int rc) {
 
 static int rs_grid(void) { return (int)fafi_cus(); }
 
+static size_t big_bytes(const fafh_plan *p, long cs, size_t L);
+static long big_chunk(const fafh_plan *p, size_t L, long count);
+
 /* signals per four-step chunk; per_sig = workspace bytes one signal takes.
  * A lent workspace (the serving arena, the static scratch of the
  * allocation-free call) caps the chunk at what it holds. */
@@ -65,9 +68,89 @@ size_t fafi_run_ws_bytes(const fafh_plan *p, long count) {
                sizeof(float);
     case PK_CWT:
         return fafi_cwt_ws_bytes(p, count, 1);
+    case PK_RADER_L:
+    case PK_BLUE_L: {
+        const size_t L = p->kind == PK_RADER_L ? p->n - 1 : (size_t)p->blue.m;
+        return big_bytes(p, big_chunk(p, L, count), L);
+    }
     default:
         return 0;
     }
+}
+
+/* ---- Rader and Bluestein beyond one workgroup ----
+ * Per chunk of signals: the gather (Rader: a[j] = x[g^j]) or the chirp
+ * (Bluestein: a[k] = x[k] chirp[k], zero-padded to M) into workspace plane
+ * pair A, the inner forward FFT into pair B, the product with the
+ * precomputed kernel spectrum (Rader saves each signal's DC sum first), the
+ * unnormalised inner inverse back into A, then the scatter (Rader:
+ * y[g^-j] = x0 + a[j], y[0] = x0 + sum) or the chirp out, with the plan's
+ * norm.  Reference src/faf_fft_rader.c:136-232, src/faf_bluestein.c:165-223.
+ * Workspace: two plane pairs of L = p-1 or M per signal, the DC sums, then
+ * the inner plans' own workspace. */
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static size_t big_bytes(const fafh_plan *p, long cs, size_t L) {
+    size_t a = align256((size_t)cs * L * 16), z = align256((size_t)cs * 8);
+    size_t w1 = fafi_run_ws_bytes(p->inner_f, cs), w2 = fafi_run_ws_bytes(p->inner_i, cs);
+    return a + z + (w1 > w2 ? w1 : w2);
+}
+
+static long big_chunk(const fafh_plan *p, size_t L, long count) {
+    long cs = (long)(CHUNK_BYTES / (L * 16));
+    if (cs < 1) cs = 1;
+    if (cs > count) cs = count;
+    if (p->ws_ext)
+        while (cs > 1 && big_bytes(p, cs, L) > p->ws_ext_bytes) cs = (cs + 1) / 2;
+    return cs;
+}
+
+/* runs an inner plan on lent workspace */
+static int inner_run(fafh_plan *q, void *ws, size_t wb, const float *ir, const float *ii,
+                     float *orr, float *oi, long dist, long count, void *stream) {
+    q->ws_ext = ws;
+    q->ws_ext_bytes = wb;
+    int rc = fafi_run_c2c(q, ir, ii, dist, orr, oi, dist, count, stream);
+    q->ws_ext = NULL;
+    q->ws_ext_bytes = 0;
+    return rc;
+}
+
+static int run_big(fafh_plan *p, const float *ir, const float *ii, long idist, float *orr,
+                   float *oi, long odist, long count, void *stream) {
+    const int rader = p->kind == PK_RADER_L;
+    const size_t n = p->n, L = rader ? n - 1 : (size_t)p->blue.m;
+    const long cs = big_chunk(p, L, count);
+    const size_t need = big_bytes(p, cs, L);
+    char *ws = fafi_workspace(p, need);
+    if (!ws) return -1;
+    float *ar = (float *)ws, *ai = ar + (size_t)cs * L;
+    float *br = ai + (size_t)cs * L, *bi = br + (size_t)cs * L;
+    const size_t ab = align256((size_t)cs * L * 16);
+    void *z = ws + ab;
+    char *sub = ws + ab + align256((size_t)cs * 8);
+    const size_t subb = need - ab - align256((size_t)cs * 8);
+    const long l = (long)L;
+    for (long s0 = 0; s0 < count; s0 += cs) {
+        const long c = count - s0 < cs ? count - s0 : cs;
+        const float *xr = ir + s0 * idist, *xi = ii + s0 * idist;
+        float *yr = orr + s0 * odist, *yi = oi + s0 * odist;
+        int rc = rader ? fafk_big_gather(xr, xi, idist, p->rader.rin, l, ar, ai, l, c, stream)
+                       : fafk_big_chirp_in(xr, xi, idist, p->blue.chirp, (long)n, l, ar, ai, l,
+                                           c, stream);
+        if (kerr(rc)) return -1;
+        if (inner_run(p->inner_f, sub, subb, ar, ai, br, bi, l, c, stream)) return -1;
+        if (kerr(fafk_big_mul(br, bi, l, rader ? p->rader.b : p->blue.b, l,
+                              rader ? z : NULL, c, stream)))
+            return -1;
+        if (inner_run(p->inner_i, sub, subb, br, bi, ar, ai, l, c, stream)) return -1;
+        rc = rader ? fafk_big_rader_out(ar, ai, l, p->rader.rout, l, xr, xi, idist, z, yr, yi,
+                                        odist, p->scale, c, stream)
+                   : fafk_big_chirp_out(ar, ai, l, p->blue.chirp, (long)n, yr, yi, odist,
+                                        p->scale, c, stream);
+        if (kerr(rc)) return -1;
+    }
+    return 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -85,6 +168,13 @@ int fafi_run_c2c(fafh_plan *p, const float *ir, const float *ii, long idist, flo
                                      count, p->scale, stream));
         return kerr(fafk_c2c(&p->a, p->inverse, inter, p->spb, ir, ii, idist, orr, oi, odist,
                              count, p->scale, stream));
+    case PK_RADER_L:
+    case PK_BLUE_L:
+        if (inter) {
+            fafi_err("Rader/Bluestein lengths need SPLIT layout on the device path");
+            return -1;
+        }
+        return run_big(p, ir, ii, idist, orr, oi, odist, count, stream);
     case PK_RADER:
     case PK_BLUE: {
         /* these kernels read/write split planes; interleaved buffers are

@@ -175,20 +175,22 @@ int fafi_default_spb(const fafd_fft *P, size_t extra_per_signal) {
     return spb;
 }
 
-int fafi_split_4step(size_t n, int *n1, int *n2) {
-    /* n = n1 * n2 with n1 <= n2, n1 as close to sqrt(n) as possible and both
-     * lengths runnable in one workgroup */
+/* n = n1 * n2 with n1 <= n2, n1 as close to sqrt(n) as possible and both
+ * lengths runnable in one workgroup (radix 7 allowed when allow7) */
+static int split_4step(size_t n, int allow7, int *n1, int *n2) {
     size_t best = 0;
     for (size_t a = 2; a * a <= n; ++a) {
         if (n % a) continue;
         size_t b = n / a;
-        if (fafi_radix_fits(a, 0) && fafi_radix_fits(b, 0)) best = a;
+        if (fafi_radix_fits(a, allow7) && fafi_radix_fits(b, allow7)) best = a;
     }
     if (!best) return -1;
     *n1 = (int)best;
     *n2 = (int)(n / best);
     return 0;
 }
+
+int fafi_split_4step(size_t n, int *n1, int *n2) { return split_4step(n, 0, n1, n2); }
 
 /* ------------------------------------------------------------------ */
 /* table cache: device tables keyed by (kind, n, direction), kept for   */
@@ -361,6 +363,8 @@ fafh_plan *fafi_plan_new(void) {
 void fafi_plan_free(fafh_plan *p) {
     if (!p) return;
     fafi_cwt_free(p);
+    fafi_plan_free(p->inner_f);
+    fafi_plan_free(p->inner_i);
     fafi_plan_free(p->sub_r2c);
     fafi_plan_free(p->sub_c2c);
     fafi_plan_free(p->sub_c2r);
@@ -437,12 +441,16 @@ static void *upload(const void *host, size_t bytes) {
     return d;
 }
 
+static fafh_plan *plan_inner(size_t m, int inv);
+
 static int plan_rader(fafh_plan *p, size_t n, int inv) {
     /* reference src/faf_fft_rader.c:19-134: a[j] = x[g^j], B = FFT_m(w^(g^-j));
      * here B is built in FP64 and carries the 1/m of the inner inverse */
     size_t m = n - 1, g = fafi_primitive_root(n);
     size_t ginv = (size_t)powmod_u64(g, n - 2, n);
-    if (fafi_make_fft(m, 0, 1, NULL, 0, &p->a) || fafi_make_fft(m, 1, 1, NULL, 0, &p->b))
+    const int one_wg = fafi_radix_fits(m, 1);
+    if (one_wg &&
+        (fafi_make_fft(m, 0, 1, NULL, 0, &p->a) || fafi_make_fft(m, 1, 1, NULL, 0, &p->b)))
         return -1;
     int32_t *rin = malloc(m * sizeof(int32_t)), *rout = malloc(m * sizeof(int32_t));
     double *br = malloc(m * sizeof(double)), *bi = malloc(m * sizeof(double));
@@ -473,8 +481,14 @@ static int plan_rader(fafh_plan *p, size_t n, int inv) {
     p->rader.rout = p->rader_mem[1];
     p->rader.b = p->rader_mem[2];
     p->kind = PK_RADER;
-    p->spb = fafi_default_spb(&p->a, (size_t)(n + (n >> 4) + 1));
-    if (p->spb < 1) { fafi_err("Rader length %zu exceeds one workgroup", n); return -1; }
+    p->spb = one_wg ? fafi_default_spb(&p->a, (size_t)(n + (n >> 4) + 1)) : 0;
+    if (p->spb >= 1) return 0;
+    /* beyond one workgroup: gather, inner FFTs through HBM, scatter */
+    p->inner_f = plan_inner(m, 0);
+    p->inner_i = p->inner_f ? plan_inner(m, 1) : NULL;
+    if (!p->inner_i) return -1;
+    p->kind = PK_RADER_L;
+    p->spb = 1;
     return 0;
 }
 
@@ -482,11 +496,9 @@ static int plan_blue(fafh_plan *p, size_t n, int inv) {
     /* reference src/faf_bluestein.c:19-163 with the chirp phase k^2 reduced
      * mod 2n in integers before the FP64 sin/cos */
     size_t M = fafi_next_5_smooth(2 * n - 1);
-    if (!fafi_radix_fits(M, 0)) {
-        fafi_err("Bluestein length %zu needs padded length %zu beyond one workgroup", n, M);
-        return -1;
-    }
-    if (fafi_make_fft(M, 0, 0, NULL, 0, &p->a) || fafi_make_fft(M, 1, 0, NULL, 0, &p->b))
+    const int one_wg = fafi_radix_fits(M, 0);
+    if (one_wg &&
+        (fafi_make_fft(M, 0, 0, NULL, 0, &p->a) || fafi_make_fft(M, 1, 0, NULL, 0, &p->b)))
         return -1;
     float *ch = malloc(2 * n * sizeof(float));
     double *br = calloc(M, sizeof(double)), *bi = calloc(M, sizeof(double));
@@ -518,18 +530,25 @@ static int plan_blue(fafh_plan *p, size_t n, int inv) {
     p->blue.chirp = p->blue_mem[0];
     p->blue.b = p->blue_mem[1];
     p->kind = PK_BLUE;
-    p->spb = fafi_default_spb(&p->a, 0);
-    return p->spb >= 1 ? 0 : -1;
+    p->spb = one_wg ? fafi_default_spb(&p->a, 0) : 0;
+    if (p->spb >= 1) return 0;
+    /* beyond one workgroup: chirp, inner FFTs through HBM, chirp */
+    p->inner_f = plan_inner(M, 0);
+    p->inner_i = p->inner_f ? plan_inner(M, 1) : NULL;
+    if (!p->inner_i) return -1;
+    p->kind = PK_BLUE_L;
+    p->spb = 1;
+    return 0;
 }
 
-static int plan_4step(fafh_plan *p, size_t n, int inv) {
+static int plan_4step_ex(fafh_plan *p, size_t n, int inv, int allow7) {
     int n1, n2;
-    if (fafi_split_4step(n, &n1, &n2)) {
+    if (split_4step(n, allow7, &n1, &n2)) {
         fafi_err("length %zu has no four-step split", n);
         return -1;
     }
-    if (fafi_make_fft((size_t)n1, inv, 0, NULL, 0, &p->a) ||
-        fafi_make_fft((size_t)n2, inv, 0, NULL, 0, &p->b))
+    if (fafi_make_fft((size_t)n1, inv, allow7, NULL, 0, &p->a) ||
+        fafi_make_fft((size_t)n2, inv, allow7, NULL, 0, &p->b))
         return -1;
     p->n1 = n1;
     p->n2 = n2;
@@ -544,9 +563,40 @@ static int plan_4step(fafh_plan *p, size_t n, int inv) {
         r >>= 1;
     p->cols = c;
     p->rows = r;
-    p->ct4 = fafi_ct_enabled() && fafk_ct4_supported(n1, n2);
-    p->rs = fafi_ct_enabled() && n == 32768;
+    p->ct4 = !allow7 && fafi_ct_enabled() && fafk_ct4_supported(n1, n2);
+    p->rs = !allow7 && fafi_ct_enabled() && n == 32768;
     return 0;
+}
+
+static int plan_4step(fafh_plan *p, size_t n, int inv) { return plan_4step_ex(p, n, inv, 0); }
+
+/* The inner C2C of a Rader / Bluestein transform too long for one
+ * workgroup: split planes, unnormalised.  5-smooth lengths get the full C2C
+ * planner (specialised kernels included); lengths with a factor 7 (Rader
+ * primes with p-1 7-smooth) get runtime radix-7 passes, in one workgroup
+ * or four-step. */
+static fafh_plan *plan_inner(size_t m, int inv) {
+    fafh_plan *q = fafi_plan_new();
+    if (!q) { fafi_err("alloc failed"); return NULL; }
+    q->scale = 1.0f;
+    int rc;
+    if (fafi_is_5_smooth(m)) {
+        rc = fafi_plan_c2c(q, m, inv, 0);
+    } else if (fafi_radix_fits(m, 1)) {
+        q->n = m;
+        q->inverse = inv;
+        rc = fafi_make_fft(m, inv, 1, NULL, 0, &q->a);
+        q->kind = PK_C2C;
+        q->spb = rc ? 0 : fafi_default_spb(&q->a, 0);
+        if (!rc && q->spb < 1) rc = -1;
+    } else {
+        q->n = m;
+        q->inverse = inv;
+        rc = plan_4step_ex(q, m, inv, 1);
+        q->kind = PK_C2C_4S;
+    }
+    if (rc) { fafi_plan_free(q); return NULL; }
+    return q;
 }
 
 int fafi_plan_c2c(fafh_plan *p, size_t n, int inv, int bluestein) {

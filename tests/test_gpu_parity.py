@@ -1148,3 +1148,43 @@ def test_single_frame_stft_matches_reference_golden(golden):
         assert rel_l2(y, golden["stft1_1000_out"]) < tol(1000)
     finally:
         G.destroy(t)
+
+
+# ------------------------------------- Rader and Bluestein beyond one workgroup
+
+@pytest.mark.parametrize("n,blue", [(12289, False), (30241, False), (40961, False),
+                                    (65537, False), (10007, True), (30011, True)])
+def test_rader_bluestein_beyond_one_workgroup(n, blue):
+    """Primes whose Rader inner length p-1 (12288 in one workgroup but not
+    beside the signal; 30240 = 2^5 3^3 5 7 four-step with radix 7; 40960 and
+    65536 four-step) or Bluestein padded length M (20250, 60750) is too long
+    for one workgroup: gather / chirp, inner FFTs through HBM, scatter /
+    chirp.  Forward and inverse (ortho), a batch of 3 with padded rows,
+    against FP64; batch members equal the signal alone."""
+    say(f"case: {'bluestein' if blue else 'rader'} n={n} beyond one workgroup")
+    lib = G.L()
+    rng = np.random.default_rng(n)
+    flags = fa.FLAG_BLUESTEIN if blue else 0
+    x = cplx(rng, 3, n)
+    t = G.create("fft", n, flags=flags)
+    ti = G.create("fft", n, flags=flags, dir="inverse", norm="ortho")
+    try:
+        # padded row distance: signal i starts i * (n + 5) elements in
+        d = n + 5
+        xp = np.zeros((3, d), np.complex64)
+        xp[:, :n] = x
+        xr, xi = G.dev(xp.real.copy()), G.dev(xp.imag.copy())
+        yr, yi = G.empty((3, d)), G.empty((3, d))
+        bi = fa.buffer_split(xr.data_ptr(), xi.data_ptr(), n)
+        bo = fa.buffer_split(yr.data_ptr(), yi.data_ptr(), n)
+        fa.check(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), 3, d, d), "fwd")
+        y = (G.host(yr) + 1j * G.host(yi))[:, :n]
+        truth = truth_fft(x)
+        assert rel_l2(y, truth) < tol(n)
+        y1 = G.c2c(t, x[2:3])
+        assert np.array_equal(y1[0], y[2])
+        z = G.c2c(ti, y)
+        assert rel_l2(z, x * np.sqrt(n)) < 2 * tol(n)        # ortho inverse of unscaled fwd
+    finally:
+        G.destroy(t)
+        G.destroy(ti)
