@@ -498,9 +498,59 @@ static faf_transform *unsupported(const char *what) {
     return NULL;
 }
 
-faf_transform *faf_create_dct(const faf_config *cfg) { (void)cfg; return unsupported("DCT"); }
-faf_transform *faf_create_dst(const faf_config *cfg) { (void)cfg; return unsupported("DST"); }
-faf_transform *faf_create_mdct(const faf_config *cfg) { (void)cfg; return unsupported("MDCT"); }
+/* DCT / DST / MDCT: creation and validation as reference
+ * src/faf_core.c:721-824 (type 0 means II, REAL layout, NONE norm by
+ * default, MDCT sizes even); the transforms themselves are the ones the
+ * reference names but leaves as FFT placeholders (faf_r2r.c) */
+static faf_transform *create_r2r(const faf_config *cfg, int kind) {
+    static const char *nm[3] = {"DCT", "DST", "MDCT"};
+    if (!cfg) {
+        fafi_err("faf_create_%s: cfg is NULL", kind == 0 ? "dct" : kind == 1 ? "dst" : "mdct");
+        return NULL;
+    }
+    faf_config c = *cfg;
+    int type = 2;
+    if (kind != 2) {
+        if (c.n == 0) { fafi_err("%s size must be > 0", nm[kind]); return NULL; }
+        type = c.dct_type ? c.dct_type : 2;
+        if (type < 1 || type > 4) {
+            fafi_err("%s type must be 1-4, got %d", nm[kind], type);
+            return NULL;
+        }
+        c.dct_type = type;
+    } else if (c.n % 2 != 0 || c.n == 0) {
+        fafi_err("MDCT size must be even, got %zu", c.n);
+        return NULL;
+    }
+    if (c.layout == FAF_LAYOUT_DEFAULT) c.layout = FAF_LAYOUT_REAL;
+    if (c.norm == FAF_NORM_DEFAULT) c.norm = FAF_NORM_NONE;
+    if (c.layout != FAF_LAYOUT_REAL) {
+        fafi_err("%s needs layout REAL, got '%s'", nm[kind], faf_layout_name(c.layout));
+        return NULL;
+    }
+    if (c.norm != FAF_NORM_NONE && c.norm != FAF_NORM_ORTHO && c.norm != FAF_NORM_FORWARD) {
+        fafi_err("%s does not support norm '%s'", nm[kind], faf_norm_name(c.norm));
+        return NULL;
+    }
+    if (check_precision(c.precision, nm[kind])) return NULL;
+    if (fafi_create_ready()) return NULL;
+    faf_transform *t = new_transform();
+    if (!t) return NULL;
+    const int inv = c.dir == FAF_DIR_INVERSE;
+    if (kind == 0) t->type = (faf_transform_type)(FAF_TRANSFORM_DCT_I + (type - 1));
+    else if (kind == 1) t->type = (faf_transform_type)(FAF_TRANSFORM_DST_I + (type - 1));
+    else t->type = inv ? FAF_TRANSFORM_IMDCT : FAF_TRANSFORM_MDCT;
+    resolve_into(t, c);
+    if (fafi_plan_r2r(t->amd, kind, type, c.n, inv, c.norm)) {
+        faf_destroy_transform(t);
+        return NULL;
+    }
+    return t;
+}
+
+faf_transform *faf_create_dct(const faf_config *cfg) { return create_r2r(cfg, 0); }
+faf_transform *faf_create_dst(const faf_config *cfg) { return create_r2r(cfg, 1); }
+faf_transform *faf_create_mdct(const faf_config *cfg) { return create_r2r(cfg, 2); }
 faf_transform *faf_create_dwt(const faf_config *cfg) { (void)cfg; return unsupported("DWT"); }
 
 faf_transform *faf_create(faf_transform_type type, const faf_config *cfg) {
@@ -517,10 +567,15 @@ faf_transform *faf_create(faf_transform_type type, const faf_config *cfg) {
         fafi_err("CWT: use faf_create_cwt / faf_create_icwt");
         return NULL;
     case FAF_TRANSFORM_DCT_I: case FAF_TRANSFORM_DCT_II: case FAF_TRANSFORM_DCT_III:
-    case FAF_TRANSFORM_DCT_IV: return faf_create_dct(&c);
+    case FAF_TRANSFORM_DCT_IV:
+        c.dct_type = 1 + (int)(type - FAF_TRANSFORM_DCT_I);
+        return faf_create_dct(&c);
     case FAF_TRANSFORM_DST_I: case FAF_TRANSFORM_DST_II: case FAF_TRANSFORM_DST_III:
-    case FAF_TRANSFORM_DST_IV: return faf_create_dst(&c);
-    case FAF_TRANSFORM_MDCT: case FAF_TRANSFORM_IMDCT: return faf_create_mdct(&c);
+    case FAF_TRANSFORM_DST_IV:
+        c.dct_type = 1 + (int)(type - FAF_TRANSFORM_DST_I);
+        return faf_create_dst(&c);
+    case FAF_TRANSFORM_IMDCT: c.dir = FAF_DIR_INVERSE; return faf_create_mdct(&c);
+    case FAF_TRANSFORM_MDCT: return faf_create_mdct(&c);
     case FAF_TRANSFORM_HAAR: case FAF_TRANSFORM_DAUBECHIES4: case FAF_TRANSFORM_CDF53:
     case FAF_TRANSFORM_CDF97: case FAF_TRANSFORM_SYM4: return faf_create_dwt(&c);
     default:
@@ -537,6 +592,18 @@ faf_transform *faf_create_inverse(const faf_transform *fwd) {
     case FAF_TRANSFORM_FFT: case FAF_TRANSFORM_IFFT: return faf_create_fft(&c);
     case FAF_TRANSFORM_RFFT: case FAF_TRANSFORM_IRFFT: return faf_create_rfft(&c);
     case FAF_TRANSFORM_STFT: return faf_create_stft(&c);
+    case FAF_TRANSFORM_DCT_I: case FAF_TRANSFORM_DCT_II: case FAF_TRANSFORM_DCT_III:
+    case FAF_TRANSFORM_DCT_IV:
+        /* II <-> III; I and IV pair with themselves (dir flips the scale side) */
+        if (fwd->type == FAF_TRANSFORM_DCT_II) c.dct_type = 3;
+        else if (fwd->type == FAF_TRANSFORM_DCT_III) c.dct_type = 2;
+        return faf_create_dct(&c);
+    case FAF_TRANSFORM_DST_I: case FAF_TRANSFORM_DST_II: case FAF_TRANSFORM_DST_III:
+    case FAF_TRANSFORM_DST_IV:
+        if (fwd->type == FAF_TRANSFORM_DST_II) c.dct_type = 3;
+        else if (fwd->type == FAF_TRANSFORM_DST_III) c.dct_type = 2;
+        return faf_create_dst(&c);
+    case FAF_TRANSFORM_MDCT: case FAF_TRANSFORM_IMDCT: return faf_create_mdct(&c);
     case FAF_TRANSFORM_CWT: {
         const fafh_plan *p = fwd->amd;
         if (!p || !p->bank) { fafi_err("faf_create_inverse: CWT has no bank"); return NULL; }
@@ -710,6 +777,11 @@ static int run_icwt(const faf_transform *t, planes_t *ps, long count, long idist
 static int run_stft_single(const faf_transform *t, planes_t *ps, long count, long idist,
                            long odist, void *stream);
 
+static int run_r2r(const faf_transform *t, planes_t *ps, long count, long idist, long odist,
+                   void *stream) {
+    return fafi_run_r2r(t->amd, ps->dev[0], idist, ps->dev[1], odist, count, stream);
+}
+
 /* the body shared by faf_execute and faf_execute_batch */
 static int execute_n(const faf_transform *t, faf_buffer *out, const faf_buffer *in,
                      size_t count, size_t in_dist, size_t out_dist) {
@@ -791,6 +863,25 @@ static int execute_n(const faf_transform *t, faf_buffer *out, const faf_buffer *
         }
         planes_add(&ps, out->re, ((count - 1) * od + n) * f, 1);
         return exec_planes(t, &ps, (long)count, (long)id, (long)od, run_c2r);
+    }
+
+    if (p->kind == PK_R2R) {
+        /* REAL in, REAL out: n -> n (DCT / DST), n -> n/2 (MDCT), n/2 -> n (IMDCT) */
+        const size_t nin = (size_t)p->r2r_nin, nout = (size_t)p->r2r_nout;
+        if (!in->re || !out->re) { fafi_err("execute buffers must provide re"); return -1; }
+        if (in->layout != FAF_LAYOUT_REAL || out->layout != FAF_LAYOUT_REAL) {
+            fafi_err("%s buffers must be REAL", faf_transform_name(t->type));
+            return -1;
+        }
+        if (in->n != nin || out->n != nout) {
+            fafi_err("%s buffer n (%zu/%zu), expected %zu in and %zu out",
+                     faf_transform_name(t->type), in->n, out->n, nin, nout);
+            return -1;
+        }
+        size_t id = in_dist ? in_dist : nin, od = out_dist ? out_dist : nout;
+        planes_add(&ps, in->re, ((count - 1) * id + nin) * f, 0);
+        planes_add(&ps, out->re, ((count - 1) * od + nout) * f, 1);
+        return exec_planes(t, &ps, (long)count, (long)id, (long)od, run_r2r);
     }
 
     if (t->type == FAF_TRANSFORM_CWT || t->type == FAF_TRANSFORM_ICWT) {
@@ -908,6 +999,12 @@ int faf_execute_f32(const faf_transform *t, float *out, const float *in) {
     if (t->type == FAF_TRANSFORM_IRFFT) {
         faf_buffer i = faf_buffer_interleaved((void *)in, t->n / 2 + 1);
         faf_buffer o = faf_buffer_real(out, t->n);
+        return faf_execute(t, &o, &i);
+    }
+    if (t->amd && ((const fafh_plan *)t->amd)->kind == PK_R2R) {
+        const fafh_plan *p = t->amd;
+        faf_buffer i = faf_buffer_real((void *)in, (size_t)p->r2r_nin);
+        faf_buffer o = faf_buffer_real(out, (size_t)p->r2r_nout);
         return faf_execute(t, &o, &i);
     }
     fafi_err("faf_execute_f32: unsupported transform type %s", faf_transform_name(t->type));

@@ -39,6 +39,9 @@ const void *fafi_twiddles(size_t n, int inv);           /* device float2[n], cac
 const void *fafi_kiss_table(size_t nc, int inv);        /* device float2[nc/2+1], cached */
 int fafi_default_spb(const fafd_fft *P, size_t extra_per_signal);
 int fafi_fft64(size_t n, int inv, double *re, double *im);   /* host FP64 FFT, in place */
+/* exp(-+ 2 pi i m / n) in FP64 from the exact integer phase m mod n
+ * (inv selects +) */
+void fafi_unit_root(size_t m, size_t n, int inv, double *c, double *s);
 int fafi_split_4step(size_t n, int *n1, int *n2);
 
 
@@ -46,7 +49,8 @@ int fafi_split_4step(size_t n, int *n1, int *n2);
 typedef enum {
     PK_NONE = 0, PK_C2C, PK_C2C_4S, PK_RADER, PK_BLUE, PK_R2C, PK_C2R, PK_R2C_4S, PK_C2R_4S,
     PK_PIPE, PK_STFT, PK_CWT, PK_ICWT,
-    PK_RADER_L, PK_BLUE_L      /* Rader / Bluestein with inner FFTs beyond one workgroup */
+    PK_RADER_L, PK_BLUE_L,     /* Rader / Bluestein with inner FFTs beyond one workgroup */
+    PK_R2R                     /* DCT / DST / MDCT through a complex FFT */
 } fafh_kind;
 
 typedef struct fafh_plan {
@@ -98,6 +102,11 @@ typedef struct fafh_plan {
     /* Rader / Bluestein beyond one workgroup: the inner forward and
      * (unnormalised) inverse C2C of length p-1 or M */
     struct fafh_plan *inner_f, *inner_i;
+    /* DCT / DST / MDCT (faf_r2r.c): y[k] = Re or Im of post[k] * U[k + off],
+     * U = C2C_L(u), u[j] = pre[j] * x[j - shift] (zero outside the input) */
+    void *r2r_pre, *r2r_post;
+    long r2r_L, r2r_nin, r2r_nout;
+    int r2r_shift, r2r_off, r2r_im;
     /* caller-lent device workspace, used instead of ws while set (the
      * serving loop lends each request its own so streams never share) */
     void *ws_ext;
@@ -145,6 +154,11 @@ int fafi_run_pipe(fafh_plan *p, const float *x, long idist, float *y, long odist
                   void *stream);
 int fafi_pipe_refresh(fafh_plan *p, void *stream);
 size_t fafi_run_ws_bytes(const fafh_plan *p, long count);   /* workspace one run needs */
+
+/* DCT / DST / MDCT (faf_r2r.c): kind 0 DCT, 1 DST (type 1..4), 2 MDCT */
+int fafi_plan_r2r(fafh_plan *p, int kind, int type, size_t n, int inverse, int norm);
+int fafi_run_r2r(fafh_plan *p, const float *x, long idist, float *y, long odist, long count,
+                 void *stream);
 
 /* CWT bank (faf_cwt.c) */
 int fafi_cwt_build(fafh_plan *p, const faf_cwt_config *cfg, int inverse_kind);

@@ -116,6 +116,11 @@ int fafk_big_rader_out(const float *ar, const float *ai, long adist, const int32
                        float *yr, float *yi, long ydist, float scale, long count, void *stream);
 int fafk_big_chirp_out(const float *ar, const float *ai, long adist, const void *chirp, long n,
                        float *yr, float *yi, long ydist, float scale, long count, void *stream);
+/* DCT / DST / MDCT: the steps around the complex FFT (faf_r2r.c) */
+int fafk_r2r_pre(const float *x, long xdist, long nin, const void *pre, int shift, long L,
+                 float *ur, float *ui, long udist, long count, void *stream);
+int fafk_r2r_post(const float *ur, const float *ui, long udist, const void *post, int off,
+                  int use_im, long nout, float *y, long ydist, long count, void *stream);
 /* long-segment dual inverse CWT: acc = sum_r Y_r * dual_r over interleaved
  * row spectra Y [count][rows][nb]; acc interleaved [count][nb] */
 int fafk_icwt_mulacc(const float *Y, long nb, int rows, const float *dual, long stride,

@@ -685,6 +685,62 @@ extern "C" int fafk_big_chirp_out(const float *ar, const float *ai, long adist,
     return 0;
 }
 
+/* ---- DCT / DST / MDCT around one complex FFT ----
+ * pre:  u[j] = pre[j] * x[j - shift] where 0 <= j - shift < nin, else 0
+ *       (j < L; u split planes of distance udist)
+ * post: y[k] = Re or Im of post[k] * U[k + off], k < nout */
+__global__ __launch_bounds__(256) void k_r2r_pre(const float *__restrict__ x, long xdist,
+                                                 long nin, const float2 *__restrict__ pre,
+                                                 int shift, long L, float *__restrict__ ur,
+                                                 float *__restrict__ ui, long udist, long bps) {
+    const long s = blockIdx.x / bps;
+    const long j = (blockIdx.x - s * bps) * 256 + threadIdx.x;
+    if (j >= L) return;
+    const long i = j - shift;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (i >= 0 && i < nin) {
+        const float a = x[s * xdist + i];
+        const float2 w = pre[j];
+        v = make_float2(a * w.x, a * w.y);
+    }
+    ur[s * udist + j] = v.x;
+    ui[s * udist + j] = v.y;
+}
+
+__global__ __launch_bounds__(256) void k_r2r_post(const float *__restrict__ ur,
+                                                  const float *__restrict__ ui, long udist,
+                                                  const float2 *__restrict__ post, int off,
+                                                  int use_im, long nout,
+                                                  float *__restrict__ y, long ydist, long bps) {
+    const long s = blockIdx.x / bps;
+    const long k = (blockIdx.x - s * bps) * 256 + threadIdx.x;
+    if (k >= nout) return;
+    const float2 u = make_float2(ur[s * udist + k + off], ui[s * udist + k + off]);
+    const float2 v = cmul(post[k], u);
+    y[s * ydist + k] = use_im ? v.y : v.x;
+}
+
+extern "C" int fafk_r2r_pre(const float *x, long xdist, long nin, const void *pre, int shift,
+                            long L, float *ur, float *ui, long udist, long count, void *stream) {
+    long bps;
+    if (big_grid(L, count, &bps, "k_r2r_pre")) return -1;
+    hipLaunchKernelGGL(k_r2r_pre, dim3((unsigned)(bps * count)), dim3(256), 0, S(stream), x, xdist,
+                       nin, (const float2 *)pre, shift, L, ur, ui, udist, bps);
+    LAUNCH_CHECK("k_r2r_pre");
+    return 0;
+}
+
+extern "C" int fafk_r2r_post(const float *ur, const float *ui, long udist, const void *post,
+                             int off, int use_im, long nout, float *y, long ydist, long count,
+                             void *stream) {
+    long bps;
+    if (big_grid(nout, count, &bps, "k_r2r_post")) return -1;
+    hipLaunchKernelGGL(k_r2r_post, dim3((unsigned)(bps * count)), dim3(256), 0, S(stream), ur, ui,
+                       udist, (const float2 *)post, off, use_im, nout, y, ydist, bps);
+    LAUNCH_CHECK("k_r2r_post");
+    return 0;
+}
+
 /* Dual-frame accumulation of the long-segment inverse CWT (reference
  * src/faf_cwt.c:958-977): acc[k] = sum over rows r, in row order, of
  * Y_r[k] * dual_r[k], with the imaginary parts of DC and Nyquist cleared.

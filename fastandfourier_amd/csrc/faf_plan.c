@@ -250,6 +250,10 @@ static void unit_root(size_t m, size_t n, int inv, double *c, double *s) {
     *s = inv ? ss : -ss;
 }
 
+void fafi_unit_root(size_t m, size_t n, int inv, double *c, double *s) {
+    unit_root(m, n, inv, c, s);
+}
+
 const void *fafi_twiddles(size_t n, int inv) {
     void *d = tab_get(1, n, inv);
     if (d) return d;
@@ -365,6 +369,8 @@ void fafi_plan_free(fafh_plan *p) {
     fafi_cwt_free(p);
     fafi_plan_free(p->inner_f);
     fafi_plan_free(p->inner_i);
+    fafk_free(p->r2r_pre);
+    fafk_free(p->r2r_post);
     fafi_plan_free(p->sub_r2c);
     fafi_plan_free(p->sub_c2c);
     fafi_plan_free(p->sub_c2r);

@@ -262,6 +262,104 @@ def icwt_dual64(w, psi):
     return fft64(full, inverse=True)[0].real / n
 
 
+# ------------------------------------------------- DCT / DST / MDCT (FP64)
+#
+# Direct O(N^2) sums of the textbook definitions (the reference names these
+# transforms but computes an FFT in their place, src/faf_transforms.c:
+# 195-222, so the definitions themselves are the oracle): angles from exact
+# integer phases, cos(pi m / q) with m reduced mod 2q.  Norms follow
+# scipy.fft: "none" = backward (forward unscaled), "ortho" (orthogonalised),
+# "forward".  `inverse` keeps the matrix of `typ` and moves the scale to the
+# inverse side, so inverse DCT-III(NONE) = dct3 / (2N) = idct(type 2).
+
+
+def _cs(m, q):
+    """cos and sin of pi * m / q for integer arrays m."""
+    r = np.mod(m, 2 * q).astype(np.float64)
+    a = np.pi * r / q
+    return np.cos(a), np.sin(a)
+
+
+def r2r64(x, kind, typ, norm="none", inverse=False):
+    """kind 'dct' / 'dst', typ 1..4; x real [N] (or rows [B, N])."""
+    x = np.atleast_2d(np.asarray(x, dtype=np.float64))
+    N = x.shape[-1]
+    n = np.arange(N)[None, :]
+    k = np.arange(N)[:, None]
+    w_in = np.ones(N)
+    if kind == "dct":
+        if typ == 1:
+            c, _ = _cs(k * n, N - 1)
+            A = 2.0 * c
+            A[:, 0] = 1.0
+            A[:, N - 1] = np.where(np.arange(N) % 2 == 0, 1.0, -1.0)
+            d = 2.0 * (N - 1)
+        elif typ == 2:
+            A = 2.0 * _cs(k * (2 * n + 1), 2 * N)[0]
+            d = 2.0 * N
+        elif typ == 3:
+            A = 2.0 * _cs(n * (2 * k + 1), 2 * N)[0]
+            A[:, 0] = 1.0
+            d = 2.0 * N
+        else:
+            A = 2.0 * _cs((2 * n + 1) * (2 * k + 1), 4 * N)[0]
+            d = 2.0 * N
+    else:
+        if typ == 1:
+            A = 2.0 * _cs((k + 1) * (n + 1), N + 1)[1]
+            d = 2.0 * (N + 1)
+        elif typ == 2:
+            A = 2.0 * _cs((k + 1) * (2 * n + 1), 2 * N)[1]
+            d = 2.0 * N
+        elif typ == 3:
+            A = 2.0 * _cs((n + 1) * (2 * k + 1), 2 * N)[1]
+            A[:, N - 1] = np.where(np.arange(N) % 2 == 0, 1.0, -1.0)
+            d = 2.0 * N
+        else:
+            A = 2.0 * _cs((2 * n + 1) * (2 * k + 1), 4 * N)[1]
+            d = 2.0 * N
+    w_out = np.ones(N)
+    if norm == "ortho":
+        w_out[:] = 1.0 / np.sqrt(d)
+        if kind == "dct" and typ == 1:
+            w_in[0] = w_in[N - 1] = np.sqrt(2.0)
+            w_out[0] /= np.sqrt(2.0)
+            w_out[N - 1] /= np.sqrt(2.0)
+        elif kind == "dct" and typ == 2:
+            w_out[0] /= np.sqrt(2.0)
+        elif kind == "dct" and typ == 3:
+            w_in[0] = np.sqrt(2.0)
+        elif kind == "dst" and typ == 2:
+            w_out[N - 1] /= np.sqrt(2.0)
+        elif kind == "dst" and typ == 3:
+            w_in[N - 1] = np.sqrt(2.0)
+    elif (norm == "none" and inverse) or (norm == "forward" and not inverse):
+        w_out[:] = 1.0 / d
+    return (x * w_in[None, :]) @ A.T * w_out[None, :]
+
+
+def mdct64(x, norm="none"):
+    """X[k] = sum_n x[n] cos(pi/M (n + 1/2 + M/2)(k + 1/2)), x [.., 2M]."""
+    x = np.atleast_2d(np.asarray(x, dtype=np.float64))
+    M = x.shape[-1] // 2
+    n = np.arange(2 * M)[None, :]
+    k = np.arange(M)[:, None]
+    A = _cs((2 * n + M + 1) * (2 * k + 1), 4 * M)[0]        # pi (2n+M+1)(2k+1) / (4M)
+    s = {"none": 1.0, "ortho": np.sqrt(2.0 / M), "forward": 2.0 / M}[norm]
+    return s * (x @ A.T)
+
+
+def imdct64(X, norm="none"):
+    """y[n] = (2/M) sum_k X[k] cos(pi/M (n + 1/2 + M/2)(k + 1/2)), n < 2M."""
+    X = np.atleast_2d(np.asarray(X, dtype=np.float64))
+    M = X.shape[-1]
+    n = np.arange(2 * M)[:, None]
+    k = np.arange(M)[None, :]
+    A = _cs((2 * n + M + 1) * (2 * k + 1), 4 * M)[0]
+    s = {"none": 2.0 / M, "ortho": np.sqrt(2.0 / M), "forward": 1.0}[norm]
+    return s * (X @ A.T)
+
+
 def _out_planes(cnt, nb, out):
     """re/im planes for cnt spectra: `out` (a pair of float32 arrays of at
     least cnt*nb values, reused across timings) or new ones"""

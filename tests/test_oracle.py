@@ -328,3 +328,74 @@ def test_cpu_baseline_timer_runs_the_reference():
         assert rel_l2(y, O.ref_rfft(x)) < 1e-6
     dtp, yp = O.time_rfft_batch(x, 3)
     assert rel_l2(yp, truth) < tol(4096)
+
+
+# ------------------------------------------------- DCT / DST / MDCT oracle
+
+
+@pytest.mark.parametrize("N", [2, 5, 8, 33, 100])
+@pytest.mark.parametrize("kind", ["dct", "dst"])
+@pytest.mark.parametrize("typ", [1, 2, 3, 4])
+@pytest.mark.parametrize("norm", ["none", "ortho", "forward"])
+def test_r2r_oracle_matches_scipy(N, kind, typ, norm):
+    """The oracle's direct sums against scipy.fft's dct / dst (a library
+    routine of the same definitions): forward, and the inverse side, which
+    for type T is scipy's idct / idst of the partner type (II <-> III)."""
+    sf = pytest.importorskip("scipy.fft")
+    if kind == "dct" and typ == 1 and N < 2:
+        return
+    x = np.random.default_rng(N * 10 + typ).standard_normal((2, N))
+    snorm = "backward" if norm == "none" else norm
+    fwd = (sf.dct if kind == "dct" else sf.dst)(x, type=typ, norm=snorm, axis=-1)
+    assert rel_l2(O.r2r64(x, kind, typ, norm), fwd) < 1e-12
+    partner = {1: 1, 2: 3, 3: 2, 4: 4}[typ]
+    inv = (sf.idct if kind == "dct" else sf.idst)(x, type=partner, norm=snorm, axis=-1)
+    assert rel_l2(O.r2r64(x, kind, typ, norm, inverse=True), inv) < 1e-12
+
+
+def test_r2r_oracle_closed_forms():
+    """Closed forms that do not go through scipy: DCT-II of a constant is
+    2N at k = 0 only; DCT-II of cos(pi m (2n+1)/(2N)) is N at k = m; the
+    ortho DCT-IV matrix is orthogonal and its own inverse; a swapped sign
+    or index (DST-II read as the k-th rather than (k+1)-th sine) fails."""
+    N = 16
+    y = O.r2r64(np.ones(N), "dct", 2)[0]
+    assert abs(y[0] - 2 * N) < 1e-12 and np.max(np.abs(y[1:])) < 1e-12
+    m = 5
+    x = np.cos(np.pi * m * (2 * np.arange(N) + 1) / (2 * N))
+    y = O.r2r64(x, "dct", 2)[0]
+    assert abs(y[m] - N) < 1e-12 and np.max(np.abs(np.delete(y, m))) < 1e-12
+    A = O.r2r64(np.eye(N), "dct", 4, "ortho")
+    assert np.allclose(A @ A.T, np.eye(N), atol=1e-12)
+    x = np.random.default_rng(0).standard_normal(N)
+    wrong = 2 * np.sin(np.pi * np.outer(np.arange(N), 2 * np.arange(N) + 1) / (2 * N)) @ x
+    assert rel_l2(O.r2r64(x, "dst", 2)[0], wrong) > 0.1
+
+
+def test_mdct_oracle_tdac_reconstruction():
+    """MDCT -> IMDCT with a sine (Princen-Bradley) window and 50% overlap-add
+    gives the signal back (time-domain aliasing cancellation); a wrong phase
+    offset (n + 1/2 instead of n + 1/2 + M/2) or IMDCT scale breaks it."""
+    M = 32
+    N = 2 * M
+    w = np.sin(np.pi * (np.arange(N) + 0.5) / N)
+    sig = np.random.default_rng(1).standard_normal(6 * M)
+    for norm in ("none", "ortho", "forward"):
+        out = np.zeros_like(sig)
+        for b in range(5):
+            seg = sig[b * M: b * M + N]
+            out[b * M: b * M + N] += w * O.imdct64(O.mdct64(w * seg, norm), norm)[0]
+        assert np.max(np.abs(out[M:5 * M] - sig[M:5 * M])) < 1e-12, norm
+    # the planted mistakes
+    n = np.arange(N)
+    bad = np.cos(np.pi / M * np.outer(np.arange(M) + 0.5, n + 0.5))
+    out = np.zeros_like(sig)
+    for b in range(5):
+        seg = sig[b * M: b * M + N]
+        out[b * M: b * M + N] += w * O.imdct64((bad @ (w * seg))[None])[0]
+    assert np.max(np.abs(out[M:5 * M] - sig[M:5 * M])) > 0.1
+    out = np.zeros_like(sig)
+    for b in range(5):
+        seg = sig[b * M: b * M + N]
+        out[b * M: b * M + N] += w * O.imdct64(O.mdct64(w * seg))[0] / 2
+    assert np.max(np.abs(out[M:5 * M] - sig[M:5 * M])) > 0.1
