@@ -50,6 +50,7 @@ static void rt_init(void) {
     if (!g_rt_rc) g_rt_rc = fafk_ct_init();
     if (!g_rt_rc) g_rt_rc = fafk_rs_init();
     if (!g_rt_rc) g_rt_rc = fafk_rg_init();
+    if (!g_rt_rc) g_rt_rc = fafk64_init();
     if (g_rt_rc) {
         snprintf(g_rt_msg, sizeof g_rt_msg, "%s", fafk_last_error());
         return;
@@ -318,14 +319,30 @@ static void resolve_into(faf_transform *t, faf_config c) {
     t->levels = c.levels;
 }
 
-static int check_precision(faf_precision p, const char *what) {
+/* FP32 everywhere; FP64 for the transforms that have the FP64 variant
+ * (FFT, RFFT, DCT/DST/MDCT: fp64_ok) */
+static int check_precision_ex(faf_precision p, const char *what, int fp64_ok) {
     if (p == FAF_PREC_FP32) return 0;
-    if (p == FAF_PREC_FP64)
-        fafi_err("%s: this build computes in FP32 only (FP64 execution is not on the "
-                 "device path)", what);
-    else
-        fafi_err("%s supports FP32 and FP64 only", what);
+    if (p == FAF_PREC_FP64) {
+        if (fp64_ok) return 0;
+        fafi_err("%s: FP64 execution covers FFT, RFFT and DCT/DST/MDCT; this transform "
+                 "computes in FP32", what);
+        return -1;
+    }
+    fafi_err("%s supports FP32 and FP64 only", what);
     return -1;
+}
+
+static int check_precision(faf_precision p, const char *what) {
+    return check_precision_ex(p, what, 0);
+}
+
+static double norm_scale64(size_t n, faf_norm norm, int inverse) {
+    switch (norm) {
+    case FAF_NORM_ORTHO: return 1.0 / sqrt((double)n);
+    case FAF_NORM_FORWARD: return inverse ? 1.0 : 1.0 / (double)n;
+    default: return inverse ? 1.0 / (double)n : 1.0;
+    }
 }
 
 /* FAF_FLAG_MEASURE (reference src/faf_core.c:539-565, called at :694):
@@ -367,7 +384,8 @@ faf_transform *faf_create_fft(const faf_config *cfg) {
         fafi_err("FFT does not support norm '%s'", faf_norm_name(c.norm));
         return NULL;
     }
-    if (check_precision(c.precision, rader ? "Rader" : blue ? "Bluestein" : "FFT")) return NULL;
+    if (check_precision_ex(c.precision, rader ? "Rader" : blue ? "Bluestein" : "FFT", 1))
+        return NULL;
     if (fafi_create_ready()) return NULL;
     faf_transform *t = new_transform();
     if (!t) return NULL;
@@ -377,6 +395,14 @@ faf_transform *faf_create_fft(const faf_config *cfg) {
     resolve_into(t, c);
     fafh_plan *p = t->amd;
     p->interleaved = c.layout == FAF_LAYOUT_INTERLEAVED;
+    if (c.precision == FAF_PREC_FP64) {
+        if (fafi_plan_c2c64(p, n, inv, blue, 0)) {
+            faf_destroy_transform(t);
+            return NULL;
+        }
+        p->scale64 = norm_scale64(n, c.norm, inv);
+        return t;
+    }
     if (fafi_plan_c2c(p, n, inv, blue)) {
         faf_destroy_transform(t);
         return NULL;
@@ -404,7 +430,7 @@ faf_transform *faf_create_rfft(const faf_config *cfg) {
                  faf_get_recommended_size(FAF_TRANSFORM_RFFT, c.n));
         return NULL;
     }
-    if (check_precision(c.precision, "R2C")) return NULL;
+    if (check_precision_ex(c.precision, "R2C", 1)) return NULL;
     if (c.layout == FAF_LAYOUT_DEFAULT) c.layout = FAF_LAYOUT_HERMITIAN;
     if (c.norm == FAF_NORM_DEFAULT) c.norm = FAF_NORM_NONE;
     if (c.layout != FAF_LAYOUT_HERMITIAN && c.layout != FAF_LAYOUT_INTERLEAVED) {
@@ -429,6 +455,14 @@ faf_transform *faf_create_rfft(const faf_config *cfg) {
     t->flags |= FAF_FLAG_REAL;
     fafh_plan *p = t->amd;
     p->interleaved = c.layout == FAF_LAYOUT_INTERLEAVED;
+    if (c.precision == FAF_PREC_FP64) {
+        if (fafi_plan_real64(p, c.n, inv)) {
+            faf_destroy_transform(t);
+            return NULL;
+        }
+        p->scale64 = norm_scale64(c.n, c.norm, inv);
+        return t;
+    }
     if (fafi_plan_real(p, c.n, inv)) {
         faf_destroy_transform(t);
         return NULL;
@@ -447,6 +481,7 @@ faf_transform *faf_create_stft(const faf_config *cfg) {
      * error in the reference) selects the framed real STFT of faf_amd.h. */
     if (!cfg) { fafi_err("faf_create_stft: cfg is NULL"); return NULL; }
     faf_config c = *cfg;
+    if (check_precision(c.precision, "STFT")) return NULL;
     if (c.win_length == 0) c.win_length = c.n;
     if (c.hop_length == 0) c.hop_length = c.win_length / 4;
     if (c.win_length > c.n) {
@@ -532,7 +567,7 @@ static faf_transform *create_r2r(const faf_config *cfg, int kind) {
         fafi_err("%s does not support norm '%s'", nm[kind], faf_norm_name(c.norm));
         return NULL;
     }
-    if (check_precision(c.precision, nm[kind])) return NULL;
+    if (check_precision_ex(c.precision, nm[kind], 1)) return NULL;
     if (fafi_create_ready()) return NULL;
     faf_transform *t = new_transform();
     if (!t) return NULL;
@@ -541,7 +576,7 @@ static faf_transform *create_r2r(const faf_config *cfg, int kind) {
     else if (kind == 1) t->type = (faf_transform_type)(FAF_TRANSFORM_DST_I + (type - 1));
     else t->type = inv ? FAF_TRANSFORM_IMDCT : FAF_TRANSFORM_MDCT;
     resolve_into(t, c);
-    if (fafi_plan_r2r(t->amd, kind, type, c.n, inv, c.norm)) {
+    if (fafi_plan_r2r(t->amd, kind, type, c.n, inv, c.norm, c.precision == FAF_PREC_FP64)) {
         faf_destroy_transform(t);
         return NULL;
     }
@@ -733,6 +768,13 @@ static int exec_planes(const faf_transform *t, planes_t *ps, long count, long id
 static int run_c2c(const faf_transform *t, planes_t *ps, long count, long idist, long odist,
                    void *stream) {
     fafh_plan *p = t->amd;
+    if (p->fp64) {
+        if (p->interleaved)
+            return fafi_run_c2c64(p, ps->dev[0], NULL, idist, ps->dev[1], NULL, odist, count,
+                                  stream);
+        return fafi_run_c2c64(p, ps->dev[0], ps->dev[1], idist, ps->dev[2], ps->dev[3], odist,
+                              count, stream);
+    }
     if (p->interleaved)
         return fafi_run_c2c(p, ps->dev[0], NULL, idist, ps->dev[1], NULL, odist, count, stream);
     return fafi_run_c2c(p, ps->dev[0], ps->dev[1], idist, ps->dev[2], ps->dev[3], odist, count,
@@ -742,6 +784,9 @@ static int run_c2c(const faf_transform *t, planes_t *ps, long count, long idist,
 static int run_r2c(const faf_transform *t, planes_t *ps, long count, long idist, long odist,
                    void *stream) {
     fafh_plan *p = t->amd;
+    if (p->fp64)
+        return fafi_run_r2c64(p, ps->dev[0], idist, ps->dev[1],
+                              p->interleaved ? NULL : ps->dev[2], odist, count, stream);
     return fafi_run_r2c(p, ps->dev[0], idist, ps->dev[1], p->interleaved ? NULL : ps->dev[2],
                         odist, count, stream);
 }
@@ -749,6 +794,12 @@ static int run_r2c(const faf_transform *t, planes_t *ps, long count, long idist,
 static int run_c2r(const faf_transform *t, planes_t *ps, long count, long idist, long odist,
                    void *stream) {
     fafh_plan *p = t->amd;
+    if (p->fp64) {
+        if (p->interleaved)
+            return fafi_run_c2r64(p, ps->dev[0], NULL, idist, ps->dev[1], odist, count, stream);
+        return fafi_run_c2r64(p, ps->dev[0], ps->dev[1], idist, ps->dev[2], odist, count,
+                              stream);
+    }
     if (p->interleaved)
         return fafi_run_c2r(p, ps->dev[0], NULL, idist, ps->dev[1], odist, count, stream);
     return fafi_run_c2r(p, ps->dev[0], ps->dev[1], idist, ps->dev[2], odist, count, stream);
@@ -779,7 +830,9 @@ static int run_stft_single(const faf_transform *t, planes_t *ps, long count, lon
 
 static int run_r2r(const faf_transform *t, planes_t *ps, long count, long idist, long odist,
                    void *stream) {
-    return fafi_run_r2r(t->amd, ps->dev[0], idist, ps->dev[1], odist, count, stream);
+    fafh_plan *p = t->amd;
+    if (p->fp64) return fafi_run_r2r64(p, ps->dev[0], idist, ps->dev[1], odist, count, stream);
+    return fafi_run_r2r(p, ps->dev[0], idist, ps->dev[1], odist, count, stream);
 }
 
 /* the body shared by faf_execute and faf_execute_batch */
@@ -793,7 +846,7 @@ static int execute_n(const faf_transform *t, faf_buffer *out, const faf_buffer *
     planes_t ps;
     memset(&ps, 0, sizeof ps);
     size_t n = t->n;
-    const size_t f = sizeof(float);
+    const size_t f = p->fp64 ? sizeof(double) : sizeof(float);
 
     if (t->type == FAF_TRANSFORM_PIPELINE) {
         if (in->layout != FAF_LAYOUT_REAL || in->n != n || !in->re) {
@@ -981,6 +1034,10 @@ int faf_execute_batch_noalloc(const faf_transform *t, faf_buffer *out, const faf
 int faf_execute_f32(const faf_transform *t, float *out, const float *in) {
     /* interleaved convenience wrapper (reference src/faf_vm.c:976-1083) */
     if (!t || !out || !in) return -1;
+    if (t->amd && ((const fafh_plan *)t->amd)->fp64) {
+        fafi_err("faf_execute_f32: the transform is FP64; use faf_execute with double buffers");
+        return -1;
+    }
     if (t->type == FAF_TRANSFORM_FFT || t->type == FAF_TRANSFORM_IFFT ||
         t->type == FAF_TRANSFORM_STFT) {
         if (t->cfg.layout != FAF_LAYOUT_INTERLEAVED) {

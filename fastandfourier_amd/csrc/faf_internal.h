@@ -50,7 +50,9 @@ typedef enum {
     PK_NONE = 0, PK_C2C, PK_C2C_4S, PK_RADER, PK_BLUE, PK_R2C, PK_C2R, PK_R2C_4S, PK_C2R_4S,
     PK_PIPE, PK_STFT, PK_CWT, PK_ICWT,
     PK_RADER_L, PK_BLUE_L,     /* Rader / Bluestein with inner FFTs beyond one workgroup */
-    PK_R2R                     /* DCT / DST / MDCT through a complex FFT */
+    PK_R2R,                    /* DCT / DST / MDCT through a complex FFT */
+    /* the FP64 variant (faf_fp64.c) */
+    PK_C2C64, PK_C2C64_4S, PK_RADER64, PK_BLUE64, PK_R2C64, PK_C2R64
 } fafh_kind;
 
 typedef struct fafh_plan {
@@ -105,6 +107,12 @@ typedef struct fafh_plan {
     /* DCT / DST / MDCT (faf_r2r.c): y[k] = Re or Im of post[k] * U[k + off],
      * U = C2C_L(u), u[j] = pre[j] * x[j - shift] (zero outside the input) */
     void *r2r_pre, *r2r_post;
+    /* FP64 plans: computed in double; tw64 / tw64b the double2 twiddle
+     * tables of a / b (R2C / C2R: the Kiss table), twn64 the four-step
+     * W_n table, scale64 the norm */
+    int fp64;
+    const void *tw64, *tw64b, *twn64;
+    double scale64;
     long r2r_L, r2r_nin, r2r_nout;
     int r2r_shift, r2r_off, r2r_im;
     /* caller-lent device workspace, used instead of ws while set (the
@@ -156,9 +164,22 @@ int fafi_pipe_refresh(fafh_plan *p, void *stream);
 size_t fafi_run_ws_bytes(const fafh_plan *p, long count);   /* workspace one run needs */
 
 /* DCT / DST / MDCT (faf_r2r.c): kind 0 DCT, 1 DST (type 1..4), 2 MDCT */
-int fafi_plan_r2r(fafh_plan *p, int kind, int type, size_t n, int inverse, int norm);
+int fafi_plan_r2r(fafh_plan *p, int kind, int type, size_t n, int inverse, int norm, int fp64);
 int fafi_run_r2r(fafh_plan *p, const float *x, long idist, float *y, long odist, long count,
                  void *stream);
+
+/* FP64 variant (faf_fp64.c); buffers hold doubles, dists in elements */
+int fafi_plan_c2c64(fafh_plan *p, size_t n, int inv, int bluestein, int allow7);
+int fafi_plan_real64(fafh_plan *p, size_t n, int inv);
+size_t fafi_ws64_bytes(const fafh_plan *p, long count);
+int fafi_run_c2c64(fafh_plan *p, const double *ir, const double *ii, long idist, double *orr,
+                   double *oi, long odist, long count, void *stream);
+int fafi_run_r2c64(fafh_plan *p, const double *x, long idist, double *orr, double *oi,
+                   long odist, long count, void *stream);
+int fafi_run_c2r64(fafh_plan *p, const double *ir, const double *ii, long idist, double *y,
+                   long odist, long count, void *stream);
+int fafi_run_r2r64(fafh_plan *p, const double *x, long idist, double *y, long odist, long count,
+                   void *stream);
 
 /* CWT bank (faf_cwt.c) */
 int fafi_cwt_build(fafh_plan *p, const faf_cwt_config *cfg, int inverse_kind);

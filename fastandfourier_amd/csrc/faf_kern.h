@@ -205,6 +205,42 @@ int fafk_ct4(int in_kind, int out_kind, int inv, const fafd_fft *P1, const fafd_
 enum { FAFK_FS_IN_SPLIT = 0, FAFK_FS_IN_PAIR = 1, FAFK_FS_IN_INTER = 2 };
 enum { FAFK_FS_OUT_SPLIT = 0, FAFK_FS_OUT_PAIR = 1, FAFK_FS_OUT_INTER = 2 };
 
+/* ---- the FP64 variant (faf_kernels64.hip) ----
+ * strided accessor: element i of transform s at (s / G) * gdist +
+ * (s % G) * tdist + i * estride complex elements; im == NULL: interleaved
+ * double2 at re */
+typedef struct fafk_acc64 {
+    const void *re, *im;
+    long G, gdist, tdist, estride;
+} fafk_acc64;
+const char *fafk64_last_error(void);
+int fafk64_init(void);
+int fafk64_threads(int n);
+/* count transforms of length n (<= 8192) of the plan record Pdev (device
+ * copy) with double2 twiddles tw; lane_major interleaves transforms lane by
+ * lane (strided columns); twn (optional) multiplies output k of transform s
+ * by twn[(k * (s % out->G)) % twN] (four-step column pass) */
+int fafk64_fft(const void *Pdev, int n, const void *tw, int inv, long count, int lane_major,
+               const fafk_acc64 *in, const fafk_acc64 *out, const void *twn, long twN,
+               double scale, void *stream);
+int fafk64_post(const void *Z, long zdist, const void *tw, long nc, double *xr, double *xi,
+                long xdist, double scale, long count, void *stream);
+int fafk64_pre(const double *xr, const double *xi, long xdist, const void *tw, long nc, void *Z,
+               long zdist, long count, void *stream);
+int fafk64_in(const double *xr, const double *xi, long xdist, const int32_t *idx,
+              const void *chirp, long n, long L, double *ar, double *ai, long adist, long count,
+              void *stream);
+int fafk64_mul(double *ar, double *ai, long adist, const void *B, long m, void *save0,
+               long count, void *stream);
+int fafk64_out(const double *ar, const double *ai, long adist, const int32_t *idx,
+               const void *chirp, long len, const double *xr, const double *xi, long xdist,
+               const void *save0, double *yr, double *yi, long ydist, double scale, long count,
+               void *stream);
+int fafk64_r2r_pre(const double *x, long xdist, long nin, const void *pre, int shift, long L,
+                   double *ur, double *ui, long udist, long count, void *stream);
+int fafk64_r2r_post(const double *ur, const double *ui, long udist, const void *post, int off,
+                    int use_im, long nout, double *y, long ydist, long count, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

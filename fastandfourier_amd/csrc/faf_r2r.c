@@ -34,12 +34,17 @@
 #include <stdlib.h>
 #include <string.h>
 
-static void *upload_c(const double *re, const double *im, size_t n) {
-    float *h = malloc(2 * n * sizeof(float));
+/* a complex table as float2 (FP32 plans) or double2 (FP64 plans) */
+static void *upload_c(const double *re, const double *im, size_t n, int fp64) {
+    const size_t es = fp64 ? sizeof(double) : sizeof(float);
+    void *h = malloc(2 * n * es);
     if (!h) { fafi_err("r2r: alloc failed"); return NULL; }
-    for (size_t i = 0; i < n; ++i) { h[2 * i] = (float)re[i]; h[2 * i + 1] = (float)im[i]; }
-    void *d = fafk_malloc(2 * n * sizeof(float));
-    if (!d || fafk_h2d(d, h, 2 * n * sizeof(float), NULL) || fafk_sync(NULL)) {
+    for (size_t i = 0; i < n; ++i) {
+        if (fp64) { ((double *)h)[2 * i] = re[i]; ((double *)h)[2 * i + 1] = im[i]; }
+        else { ((float *)h)[2 * i] = (float)re[i]; ((float *)h)[2 * i + 1] = (float)im[i]; }
+    }
+    void *d = fafk_malloc(2 * n * es);
+    if (!d || fafk_h2d(d, h, 2 * n * es, NULL) || fafk_sync(NULL)) {
         fafi_err("r2r: %s", fafk_last_error());
         fafk_free(d);
         d = NULL;
@@ -56,7 +61,7 @@ static void phase(long m, size_t q, int plus, double *c, double *s) {
     fafi_unit_root((size_t)r, (size_t)per, plus, c, s);
 }
 
-int fafi_plan_r2r(fafh_plan *p, int kind, int type, size_t n, int inverse, int norm) {
+int fafi_plan_r2r(fafh_plan *p, int kind, int type, size_t n, int inverse, int norm, int fp64) {
     const int ortho = norm == FAF_NORM_ORTHO, fwdn = norm == FAF_NORM_FORWARD;
     size_t N = n, L, nin, nout;
     int shift = 0, off = 0, im = 0, inner_inv = 0;
@@ -216,19 +221,22 @@ int fafi_plan_r2r(fafh_plan *p, int kind, int type, size_t n, int inverse, int n
             }
         }
     }
-    p->r2r_pre = upload_c(pr, pi, L);
-    p->r2r_post = upload_c(qr, qi, nout);
+    p->r2r_pre = upload_c(pr, pi, L, fp64);
+    p->r2r_post = upload_c(qr, qi, nout, fp64);
     free(pr); free(pi); free(qr); free(qi); free(al); free(be);
     if (!p->r2r_pre || !p->r2r_post) return -1;
     /* the inner C2C: any length (Rader / Bluestein where not 5-smooth) */
     fafh_plan *q = fafi_plan_new();
     if (!q) { fafi_err("r2r: alloc failed"); return -1; }
-    if (L == 1 || fafi_plan_c2c(q, L, inner_inv, 1)) {
+    if (L == 1 || (fp64 ? fafi_plan_c2c64(q, L, inner_inv, 1, 0)
+                        : fafi_plan_c2c(q, L, inner_inv, 1))) {
         if (L == 1) fafi_err("r2r: inner length 1");
         fafi_plan_free(q);
         return -1;
     }
     q->scale = 1.0f;
+    q->scale64 = 1.0;
+    p->fp64 = fp64;
     p->inner_f = q;
     p->kind = PK_R2R;
     p->n = n;
