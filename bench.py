@@ -41,6 +41,8 @@ def clock_warmup(fn, torch, seconds):
     steps start at steady device clocks however short --warmup is."""
     t0 = time.perf_counter()
     launches = 0
+    if seconds <= 0:
+        return {"launches": 0, "ms": 0.0}
     while True:
         for _ in range(25):
             fn()
@@ -103,6 +105,9 @@ def parse():
     ap.add_argument("--only", default="", help="comma list of extra workloads")
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline (for profiler runs of the GPU kernels)")
+    ap.add_argument("--clock-warmup", type=float, default=CLOCK_WARMUP_S,
+                    help="seconds of headline launches before the counted warm-up "
+                         "(counter runs, which serialise dispatches, may use 0)")
     return ap.parse_args()
 
 
@@ -169,7 +174,7 @@ def main():
     # headline kernel first runs for a fixed wall time, whatever --warmup
     # is, and only then come the W counted warm-up steps and the K timed
     # steps.  This phase is reported as its own field.
-    clock_warm = clock_warmup(step_r2c, torch, CLOCK_WARMUP_S)
+    clock_warm = clock_warmup(step_r2c, torch, args.clock_warmup)
     log("clock warm-up:", clock_warm)
     ms = timed(step_r2c, args.steps, args.warmup)
     gbs = algo_bytes / (ms * 1e-3) / 1e9
@@ -267,10 +272,19 @@ def main():
     extra["plain_per_call_rfft_4096_gbs"] = round(pc * (4 * n + 8 * nb) / (ms_plain * 1e-3) / 1e9, 2)
     extra["batched_over_plain"] = round(gbs / extra["plain_per_call_rfft_4096_gbs"], 1)
 
-    # copy ceiling on this box (read + write)
+    # copy ceilings on this box (read + write of 1 GiB): torch's copy (the
+    # runtime's blit kernel) and the library's 16-byte streaming copy, at a
+    # few grid sizes (best kept)
     z = torch.empty_like(x)
     ms_copy = timed(lambda: z.copy_(x), 20, 5)
     extra["d2d_copy_gbs"] = round(2 * x.numel() * 4 / (ms_copy * 1e-3) / 1e9, 1)
+    best = 0.0
+    for g in (1024, 2048, 4096, 8192):
+        ms_c = timed(lambda: fa.check(lib.faf_amd_copy_ceiling(z.data_ptr(), x.data_ptr(),
+                                                               x.numel() * 4, g), "copy"), 20, 5)
+        best = max(best, 2 * x.numel() * 4 / (ms_c * 1e-3) / 1e9)
+    extra["copy_kernel_gbs"] = round(best, 1)
+    extra["headline_over_copy_kernel"] = round(gbs / best, 3)
     del z
     lib.faf_destroy_transform(t)
     lib.faf_destroy_transform(ti)
@@ -670,7 +684,7 @@ def run_serving(lib, fa, torch, np, extra, count=100000, rate=200000.0):
         fa.check(lib.faf_server_set_window(s, 60.0), "window")
         extra["c5_serving_window30"] = w30
         sweep = {}
-        for r in (400000.0, 800000.0, 1600000.0):
+        for r in (250000.0, 300000.0, 350000.0, 400000.0, 800000.0, 1600000.0):
             log("c5 sweep", r)
             a = (arr.astype(np.float64) * (rate / r)).astype(np.uint64)
             sweep[str(int(r))] = replay(a, count)
@@ -693,6 +707,27 @@ def run_serving(lib, fa, torch, np, extra, count=100000, rate=200000.0):
         extra["c5_requests"] = count
     finally:
         lib.faf_server_destroy(s)
+    # cold plans: a fresh server with nothing prepared replays the first
+    # 5000 requests of the stream at 200k req/s; every first use of a
+    # (kind, length) builds and uploads its plan inline, on the submitting
+    # thread, and the requests behind it wait
+    try:
+        cold_n = 5000
+        s2 = lib.faf_server_create(C.byref(cfg))
+        assert s2, lib.faf_get_error()
+        lib.faf_server_stats_reset(s2)
+        t0 = time.perf_counter()
+        fa.check(lib.faf_server_replay(s2, kinds.ctypes.data, ns.ctypes.data, ins.ctypes.data,
+                                       arr.ctypes.data, cold_n, fa.REQ_DISCARD, None), "cold")
+        wall = time.perf_counter() - t0
+        lib.faf_server_stats_get(s2, C.byref(st))
+        extra["c5_cold_plans"] = {
+            "requests": cold_n, "distinct_plans": len({(int(k), int(n)) for k, n in
+                                                       zip(kinds[:cold_n], ns[:cold_n])}),
+            "p50_us": round(st.p50_us, 1), "p99_us": round(st.p99_us, 1),
+            "max_us": round(st.max_us, 1), "wall_s": round(wall, 2)}
+        lib.faf_server_destroy(s2)
+    finally:
         del pool
         torch.cuda.empty_cache()
 

@@ -190,6 +190,17 @@ int faf_amd_synchronize(void) {
 
 int faf_amd_device_ready(void) { return fafi_runtime_ready(); }
 
+int faf_amd_copy_ceiling(void *dst, const void *src, size_t bytes, int grid) {
+    if (!dst || !src) { fafi_err("copy: null buffer"); return -1; }
+    if (fafi_runtime_ready()) return -1;
+    if (grid <= 0) grid = (int)(8 * fafi_cus());
+    void *stream = fafi_stream();
+    int rc = fafk_copy16(dst, src, bytes, grid, stream);
+    if (!rc && fafi_sync_mode()) rc = fafk_sync(stream);
+    if (rc) fafi_err("%s", fafk_last_error());
+    return rc;
+}
+
 /* ------------------------------------------------------------------ */
 /* lifecycle and names (reference src/faf_core.c:27-68, 222-307)        */
 /* ------------------------------------------------------------------ */

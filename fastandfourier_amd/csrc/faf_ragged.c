@@ -550,22 +550,30 @@ int faf_ragged_autotune(faf_ragged *r) {
     }
     if (!rc) {
         fafi_err("%s", "");     /* candidates a length cannot take left messages */
-        /* whole-batch decision: the per-length winners (which may need the
-         * runtime paths, and with them the larger kernel) against the best
-         * compile-time choice of every length that has one */
+        /* whole-batch decision, timed on the batch itself: the per-length
+         * winners (which may need the runtime paths, and with them the
+         * larger kernel), the best compile-time choice of every length that
+         * has one, and the planner's defaults -- so a batch never ends up
+         * slower than untuned because single-length timings mislead */
         rg_overrides ov_best = {uniq, best, nu};
         rg_choice *lean = malloc((size_t)nu * sizeof(rg_choice));
-        for (int k = 0; k < nu; ++k) lean[k] = has_ct[k] ? best_ct[k] : best[k];
-        rg_overrides ov_lean = {uniq, lean, nu};
-        faf_ragged *a = ragged_new(r->lengths, r->count, r->flags, &ov_best);
-        faf_ragged *b = ragged_new(r->lengths, r->count, r->flags, &ov_lean);
-        float ta = a ? time_ragged(a, buf, stream) : -1.0f;
-        float tb = b ? time_ragged(b, buf, stream) : -1.0f;
+        rg_choice *dflt = malloc((size_t)nu * sizeof(rg_choice));
+        for (int k = 0; k < nu; ++k) {
+            lean[k] = has_ct[k] ? best_ct[k] : best[k];
+            memset(&dflt[k], 0, sizeof dflt[k]);
+            dflt[k].kind = -1;
+        }
+        rg_overrides ov_lean = {uniq, lean, nu}, ov_dflt = {uniq, dflt, nu};
+        const rg_choice *cand[3] = {best, lean, dflt};
+        const rg_overrides *ovs[3] = {&ov_best, &ov_lean, &ov_dflt};
         const rg_choice *win = NULL;
-        if (ta > 0.0f && (tb <= 0.0f || ta <= tb)) win = best;
-        else if (tb > 0.0f) win = lean;
-        faf_ragged_destroy(a);
-        faf_ragged_destroy(b);
+        float twin = 1e30f;
+        for (int c = 0; c < 3; ++c) {
+            faf_ragged *a = ragged_new(r->lengths, r->count, r->flags, ovs[c]);
+            float ta = a ? time_ragged(a, buf, stream) : -1.0f;
+            faf_ragged_destroy(a);
+            if (ta > 0.0f && ta < twin) { twin = ta; win = cand[c]; }
+        }
         if (!win) {
             fafi_err("ragged autotune: the batch did not run");
             rc = -1;
@@ -577,6 +585,7 @@ int faf_ragged_autotune(faf_ragged *r) {
             if (ragged_build(r, &ov)) rc = -1;
         }
         free(lean);
+        free(dflt);
     }
     fafk_free(buf);
     free(uniq); free(cnt); free(best); free(best_ct); free(has_ct);

@@ -23,6 +23,11 @@ void *faf_amd_get_stream(void);
 int faf_amd_set_sync(int on);
 int faf_amd_synchronize(void);
 int faf_amd_device_ready(void);              /* 0 if a gfx950 device is usable */
+/* the device's streaming copy (16-byte loads and stores, grid-stride over
+ * `grid` workgroups; 0 = 8 per CU): what a read-once / write-once kernel
+ * reaches on this board, the practical ceiling the transforms are measured
+ * against beside the 8 TB/s peak.  Device buffers, 16-byte aligned. */
+int faf_amd_copy_ceiling(void *dst, const void *src, size_t bytes, int grid);
 
 /* ---- graph capture of executes (thread-local) ----
  * Between faf_graph_begin and faf_graph_end, faf_execute* and
