@@ -6,7 +6,7 @@
  * tests/test_rfft.cpp:239-261).  Here an FP64 config gets a device plan
  * that computes in double throughout: the generic double-precision
  * Stockham kernel of faf_kernels64.hip over runtime radix plans (lengths
- * up to 8192 in one workgroup, four-step beyond), FP64 twiddle tables from
+ * up to 4096 in one workgroup, four-step beyond), FP64 twiddle tables from
  * exact integer phases, the Kiss post/pre passes for R2C/C2R (the real
  * signal is read or written directly as packed complex pairs), Rader and
  * Bluestein as gather / chirp, inner FFTs, kernel product and scatter /

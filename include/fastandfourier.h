@@ -5,9 +5,10 @@
  * path: the enum values, struct field order and the static-inline helpers
  * below match include/fastandfourier.h of the reference so that a program
  * written against that header compiles and links against this library
- * unchanged.  Only the transform-execution subset is implemented; DCT/DST,
- * MDCT and DWT creators are declared for ABI completeness and return NULL
- * with an error message.
+ * unchanged.  The transform-execution subset is implemented: FFT, RFFT,
+ * STFT, CWT/ICWT, the spectral pipeline, and the DCT/DST/MDCT family built
+ * on the same device FFT.  The DWT creator is declared for ABI completeness
+ * and returns NULL with an error message.
  *
  * Execution runs on the GPU (gfx950).  faf_execute accepts host or device
  * pointers: device pointers are used in place, host buffers are staged
@@ -244,9 +245,11 @@ faf_transform *faf_create_fft(const faf_config *cfg);
 faf_transform *faf_create_rfft(const faf_config *cfg);
 faf_transform *faf_create_stft(const faf_config *cfg);
 faf_transform *faf_create_inverse(const faf_transform *fwd);
-faf_transform *faf_create_dct(const faf_config *cfg);   /* not on this path: NULL */
-faf_transform *faf_create_dst(const faf_config *cfg);   /* not on this path: NULL */
-faf_transform *faf_create_mdct(const faf_config *cfg);  /* not on this path: NULL */
+/* DCT / DST types I-IV (cfg->dct_type 1-4, 0 = II) and MDCT (n even, n -> n/2),
+   REAL layout, FP32 or FP64; reference include/fastandfourier.h:703-718 */
+faf_transform *faf_create_dct(const faf_config *cfg);
+faf_transform *faf_create_dst(const faf_config *cfg);
+faf_transform *faf_create_mdct(const faf_config *cfg);
 faf_transform *faf_create_dwt(const faf_config *cfg);   /* not on this path: NULL */
 void faf_destroy_transform(faf_transform *t);
 

@@ -399,3 +399,22 @@ def test_mdct_oracle_tdac_reconstruction():
         seg = sig[b * M: b * M + N]
         out[b * M: b * M + N] += w * O.imdct64(O.mdct64(w * seg))[0] / 2
     assert np.max(np.abs(out[M:5 * M] - sig[M:5 * M])) > 0.1
+
+
+@pytest.mark.parametrize("M", [2, 6, 16, 250])
+def test_mdct_oracle_is_folded_dct4(M):
+    """The textbook reduction of the MDCT to a DCT-IV of half the length:
+    with x = (a, b, c, d) in quarters, MDCT(x) = DCT-IV(-c' - d, a - b') / 2
+    (' = reversed; scipy's unnormalised DCT-IV carries a factor 2).  This
+    pins the oracle's direct sum on a library routine, independently of the
+    TDAC check; the IMDCT is pinned as its transpose times 2/M."""
+    sf = pytest.importorskip("scipy.fft")
+    N = 2 * M
+    x = np.random.default_rng(M).standard_normal(N)
+    h = M // 2
+    a, b, c, d = x[:h], x[h:M], x[M:M + h], x[M + h:]
+    u = np.concatenate([-c[::-1] - d, a - b[::-1]])
+    assert rel_l2(O.mdct64(x[None])[0], sf.dct(u, type=4) / 2) < 1e-12
+    A = O.mdct64(np.eye(N))                   # rows: MDCT of unit vectors
+    X = np.random.default_rng(M + 1).standard_normal(M)
+    assert rel_l2(O.imdct64(X[None])[0], (2.0 / M) * (A @ X)) < 1e-12
