@@ -73,6 +73,8 @@ typedef struct fafd_rplan {
     int32_t len;         /* signal length */
     int32_t threads;     /* threads per signal */
     int32_t spb;         /* signals per workgroup */
+    int32_t variant;     /* compile-time kinds: which pass sequence of the length */
+    int32_t path;        /* compile-time kinds: the kernel's code path, -1 otherwise */
     fafd_fft fwd;        /* main or inner forward FFT */
     fafd_fft inv;        /* inner inverse FFT (Rader / Bluestein) */
     fafd_rader rader;

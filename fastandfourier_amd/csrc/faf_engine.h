@@ -158,14 +158,66 @@ template <bool INV> FAF_DEV void dft16(float2 *v) {
     }
 }
 
+template <int R, bool INV> FAF_DEV void dft(float2 *v);
+
+/* Composite R = A B with gcd(A, B) = 1 by the prime-factor (Good-Thomas)
+ * map: input n = (B n1 + A n2) mod R, A-point DFTs over n1, B-point DFTs
+ * over n2, output k = the CRT of (k1 mod A, k2 mod B).  W_R^{nk} splits
+ * into W_A^{n1 k1} W_B^{n2 k2} exactly, so no twiddle is multiplied: every
+ * index is a compile-time constant once unrolled, and the maps are register
+ * renames. */
+template <int A, int B> struct Pfa {
+    static constexpr int R = A * B;
+    static constexpr int inv_mod(int a, int m) {
+        for (int x = 1; x < m; ++x)
+            if ((a * x) % m == 1) return x;
+        return 1;
+    }
+    /* k = k1 B (B^-1 mod A) + k2 A (A^-1 mod B)  (mod R) */
+    static constexpr int EA = B * inv_mod(B % A, A), EB = A * inv_mod(A % B, B);
+};
+
+template <int A, int B, bool INV> FAF_DEV void dft_pfa(float2 *v) {
+    using P = Pfa<A, B>;
+    float2 m[B][A];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) m[n2][n1] = v[(B * n1 + A * n2) % P::R];
+        dft<A, INV>(m[n2]);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+        float2 c[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) c[n2] = m[n2][k1];
+        dft<B, INV>(c);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) v[(k1 * P::EA + k2 * P::EB) % P::R] = c[k2];
+    }
+}
+
 template <int R, bool INV> FAF_DEV void dft(float2 *v) {
     if constexpr (R == 2) dft2<INV>(v);
     else if constexpr (R == 3) dft3<INV>(v);
     else if constexpr (R == 4) dft4<INV>(v);
     else if constexpr (R == 5) dft5<INV>(v);
+    else if constexpr (R == 6) dft_pfa<2, 3, INV>(v);
     else if constexpr (R == 7) dft7<INV>(v);
     else if constexpr (R == 8) dft8<INV>(v);
-    else dft16<INV>(v);
+    else if constexpr (R == 10) dft_pfa<2, 5, INV>(v);
+    else if constexpr (R == 12) dft_pfa<4, 3, INV>(v);
+    else if constexpr (R == 14) dft_pfa<2, 7, INV>(v);
+    else if constexpr (R == 15) dft_pfa<3, 5, INV>(v);
+    else if constexpr (R == 16) dft16<INV>(v);
+    else if constexpr (R == 20) dft_pfa<4, 5, INV>(v);
+    else if constexpr (R == 21) dft_pfa<3, 7, INV>(v);
+    else if constexpr (R == 24) dft_pfa<8, 3, INV>(v);
+    else if constexpr (R == 28) dft_pfa<4, 7, INV>(v);
+    else if constexpr (R == 30) dft_pfa<2, 15, INV>(v);
+    else if constexpr (R == 40) dft_pfa<8, 5, INV>(v);
+    else if constexpr (R == 48) dft_pfa<16, 3, INV>(v);
+    else static_assert(R == 16, "no codelet of this radix");
 }
 
 /* ------------------------------------------------------------------ */
@@ -239,7 +291,7 @@ struct GPairOut {            /* C2R unpacking y[2i] = Re z, y[2i+1] = Im z */
  * for the set bits of t (at most three products for t < 16).  The few
  * loads go out with the pass's data reads, before its barrier. */
 template <int R> struct TwBits {
-    static constexpr int K = R <= 2 ? 1 : R <= 4 ? 2 : R <= 8 ? 3 : 4;
+    static constexpr int K = R <= 2 ? 1 : R <= 4 ? 2 : R <= 8 ? 3 : R <= 16 ? 4 : R <= 32 ? 5 : 6;
 };
 
 template <int R> FAF_DEV void tw_expand(const float2 *b, float2 *w) {

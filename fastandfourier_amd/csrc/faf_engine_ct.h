@@ -160,25 +160,47 @@ template <> struct Pow2Seq<4096> { using type = Radices<16, 16, 16>; };
 template <> struct Pow2Seq<8192> { using type = Radices<16, 16, 16, 2>; };
 template <> struct Pow2Seq<16384> { using type = Radices<16, 16, 16, 4>; };
 
-/* 5-smooth lengths with compile-time passes in the ragged launch: the
- * runtime planner's radix order (powers of two first, then 3s, then 5s) and
- * its thread count, so one unit's layout is the same either way */
-template <int N> struct MixSeq { static constexpr int T = 0; };
-template <> struct MixSeq<4800> { using type = Radices<16, 4, 3, 5, 5>; static constexpr int T = 320; };
-template <> struct MixSeq<3000> { using type = Radices<8, 3, 5, 5, 5>; static constexpr int T = 200; };
-template <> struct MixSeq<1000> { using type = Radices<8, 5, 5, 5>; static constexpr int T = 67; };
-template <> struct MixSeq<360> { using type = Radices<8, 3, 3, 5>; static constexpr int T = 24; };
-template <> struct MixSeq<60> { using type = Radices<4, 3, 5>; static constexpr int T = 4; };
+/* 5-smooth lengths with compile-time passes in the ragged launch.
+ * Variant 0: the runtime planner's radix order (powers of two first, then
+ * 3s, then 5s) and its thread count.  Variant 1 (and 2): composite
+ * coprime radices (prime-factor codelets, no inner twiddles) that cut the
+ * passes, and with them the LDS round trips and barriers, from 4-5 to 2-3;
+ * the thread count is the largest butterfly count of a pass, or less where
+ * a pass takes two butterflies per thread so that more signals fill a
+ * 512-thread unit. */
+template <int N, int V = 0> struct MixSeq { static constexpr int T = 0; };
+template <> struct MixSeq<4800, 0> { using type = Radices<16, 4, 3, 5, 5>; static constexpr int T = 320; };
+template <> struct MixSeq<3000, 0> { using type = Radices<8, 3, 5, 5, 5>; static constexpr int T = 200; };
+template <> struct MixSeq<1000, 0> { using type = Radices<8, 5, 5, 5>; static constexpr int T = 67; };
+template <> struct MixSeq<360, 0> { using type = Radices<8, 3, 3, 5>; static constexpr int T = 24; };
+template <> struct MixSeq<60, 0> { using type = Radices<4, 3, 5>; static constexpr int T = 4; };
+template <> struct MixSeq<4800, 1> { using type = Radices<12, 20, 20>; static constexpr int T = 240; };
+template <> struct MixSeq<3000, 1> { using type = Radices<10, 20, 15>; static constexpr int T = 150; };
+template <> struct MixSeq<3000, 2> { using type = Radices<30, 10, 10>; static constexpr int T = 100; };
+template <> struct MixSeq<1000, 1> { using type = Radices<10, 10, 10>; static constexpr int T = 100; };
+template <> struct MixSeq<360, 1> { using type = Radices<24, 15>; static constexpr int T = 24; };
+template <> struct MixSeq<60, 1> { using type = Radices<15, 4>; static constexpr int T = 4; };
 
 /* Rader primes of the ragged launch with compile-time inner passes over
- * p - 1 points: the runtime planner's radix order for p - 1 (7s allowed) and
- * threads per signal */
-template <int P> struct RaderSeq { static constexpr int T = 0; };
-template <> struct RaderSeq<11> { using type = Radices<2, 5>; static constexpr int T = 1; };
-template <> struct RaderSeq<17> { using type = Radices<16>; static constexpr int T = 1; };
-template <> struct RaderSeq<31> { using type = Radices<2, 3, 5>; static constexpr int T = 2; };
-template <> struct RaderSeq<97> { using type = Radices<16, 2, 3>; static constexpr int T = 7; };
-template <> struct RaderSeq<1009> { using type = Radices<16, 3, 3, 7>; static constexpr int T = 72; };
+ * p - 1 points.  Variant 0: the runtime planner's radix order for p - 1
+ * (7s allowed) and threads per signal; variant 1: composite radices. */
+template <int P, int V = 0> struct RaderSeq { static constexpr int T = 0; };
+template <> struct RaderSeq<11, 0> { using type = Radices<2, 5>; static constexpr int T = 1; };
+template <> struct RaderSeq<17, 0> { using type = Radices<16>; static constexpr int T = 1; };
+template <> struct RaderSeq<31, 0> { using type = Radices<2, 3, 5>; static constexpr int T = 2; };
+template <> struct RaderSeq<97, 0> { using type = Radices<16, 2, 3>; static constexpr int T = 7; };
+template <> struct RaderSeq<1009, 0> { using type = Radices<16, 3, 3, 7>; static constexpr int T = 72; };
+template <> struct RaderSeq<11, 1> { using type = Radices<10>; static constexpr int T = 1; };
+template <> struct RaderSeq<31, 1> { using type = Radices<6, 5>; static constexpr int T = 6; };
+template <> struct RaderSeq<97, 1> { using type = Radices<8, 12>; static constexpr int T = 12; };
+template <> struct RaderSeq<1009, 1> { using type = Radices<12, 12, 7>; static constexpr int T = 84; };
+
+/* the radices of a sequence into o[], returning their count */
+template <int... Rs> constexpr int radices_out(Radices<Rs...>, int *o) {
+    int k = 0;
+    ((o[k++] = Rs), ...);
+    return k;
+}
 
 template <bool INV, int N, int T, class Seq> struct MixFFT;
 template <bool INV, int N, int T, int... Rs> struct MixFFT<INV, N, T, Radices<Rs...>> {

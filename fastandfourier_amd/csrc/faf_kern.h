@@ -150,12 +150,16 @@ int fafk_blue(const fafd_fft *Pf, const fafd_fft *Pi, const fafd_blue *Bq, int s
 int fafk_rg_init(void);   /* faf_kernels_rg.hip */
 const char *fafk_rg_last_error(void);
 int fafk_rg_ct_slots(int n);
-int fafk_rg_ct_threads(int n);
-int fafk_rg_rader_threads(int p);
-int fafk_ragged(const void *plans, const void *units, int nunits, const int32_t *order,
-                const long *offsets, const float *in_re, const float *in_im, float *out_re,
-                float *out_im, unsigned *counter, int grid, int block, size_t lds,
-                int generic, void *stream);
+/* compile-time sequences of the ragged launch: threads per signal of
+ * length n (Rader: prime n) in variant v, 0 if none; radices into order */
+int fafk_rg_ct_seq(int n, int v, int *order, int *nr);
+int fafk_rg_rader_seq(int p, int v, int *order, int *nr);
+int fafk_rg_path(int kind, int n, int v);
+int fafk_ragged(const void *plans, const void *units, int nunits, const long *uoff,
+                const float *in_re, const float *in_im, float *out_re, float *out_im,
+                unsigned *counter, int grid, int block, size_t lds, int generic, int persist,
+                void *stream);
+int fafk_ragged_counter_init(unsigned *counter, int grid, void *stream);
 
 int fafk_serve(const void *plans, const void *units, int nunits, const void *sigs,
                unsigned *counter, int grid, int block, size_t lds, void *stream);

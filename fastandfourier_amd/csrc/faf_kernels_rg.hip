@@ -42,21 +42,21 @@ FAF_DEV void rg_ct(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &d
     ch.run_lazy(src, lds, dst, lt, active, (const float2 *)pl->fwd.tw);
 }
 
-template <int N>
+template <int N, int V>
 FAF_DEV void rg_mix(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst, int sid,
                     int lt, int active) {
     Lds lds{faf_smem + sid * pl->fwd.plane};
-    typename MixFFT<false, N, MixSeq<N>::T, typename MixSeq<N>::type>::Chain ch;
+    typename MixFFT<false, N, MixSeq<N, V>::T, typename MixSeq<N, V>::type>::Chain ch;
     ch.run_lazy(src, lds, dst, lt, active, (const float2 *)pl->fwd.tw);
 }
 
 /* Rader for prime P with compile-time passes over the P - 1 inner points;
  * the same steps as rader_body (faf_engine.h) */
-template <int P>
+template <int P, int V>
 FAF_DEV void rg_rader(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst,
                       int sid, int lt, int active) {
-    constexpr int T = RaderSeq<P>::T;
-    using Seq = typename RaderSeq<P>::type;
+    constexpr int T = RaderSeq<P, V>::T;
+    using Seq = typename RaderSeq<P, V>::type;
     const fafd_rader &R = pl->rader;
     const int per = pl->fwd.plane + lpad(P) + 1;
     Lds xa{faf_smem + sid * per};
@@ -119,54 +119,53 @@ FAF_DEV void rg_rs8(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &
     }
 }
 
-template <int MT, bool GEN>
-__global__ __launch_bounds__(MT, GEN ? 1 : 4) void kg_ragged(const fafd_rplan *plans, const fafd_unit *units,
-                                                int nunits, const int32_t *order,
-                                                const long *offsets, const float *in_re,
-                                                const float *in_im, float *out_re,
-                                                float *out_im, unsigned *counter) {
-    __shared__ int s_unit;
-    const int tid = (int)threadIdx.x;
-    for (;;) {
-        if (tid == 0) s_unit = (int)atomicAdd(counter, 1u);
-        __syncthreads();
-        const int u = s_unit;
-        __syncthreads();
-        if (u >= nunits) break;
-        const fafd_unit un = units[u];
-        const fafd_rplan *pl = plans + un.plan;
-        const int T = pl->threads;
-        const int sid = tid / T, lt = tid - sid * T;
-        const int active = sid < un.count;
-        const int s = active ? order[un.first + sid] : order[un.first];
-        const long off = offsets[s];
-        GSplitIn src{in_re + off, in_im + off};
-        GSplitOut dst{out_re + off, out_im + off, 1.0f};
-        if (pl->kind == FAFD_KIND_CT) {
-            switch (pl->len) {
-            case 256: rg_ct<256>(pl, src, dst, sid, lt, active); break;
-            case 512: rg_ct<512>(pl, src, dst, sid, lt, active); break;
-            case 1024: rg_ct<1024>(pl, src, dst, sid, lt, active); break;
-            case 2048: rg_ct<2048>(pl, src, dst, sid, lt, active); break;
-            case 4096: rg_ct<4096>(pl, src, dst, sid, lt, active); break;
-            case 8192: rg_rs8(pl, src, dst, sid, lt, active); break;
-            case 60: rg_mix<60>(pl, src, dst, sid, lt, active); break;
-            case 360: rg_mix<360>(pl, src, dst, sid, lt, active); break;
-            case 1000: rg_mix<1000>(pl, src, dst, sid, lt, active); break;
-            case 3000: rg_mix<3000>(pl, src, dst, sid, lt, active); break;
-            case 4800: rg_mix<4800>(pl, src, dst, sid, lt, active); break;
-            default: break;             /* the host sets this kind for the lengths above only */
-            }
-        } else if (pl->kind == FAFD_KIND_RADER_CT) {
-            switch (pl->len) {
-            case 11: rg_rader<11>(pl, src, dst, sid, lt, active); break;
-            case 17: rg_rader<17>(pl, src, dst, sid, lt, active); break;
-            case 31: rg_rader<31>(pl, src, dst, sid, lt, active); break;
-            case 97: rg_rader<97>(pl, src, dst, sid, lt, active); break;
-            case 1009: rg_rader<1009>(pl, src, dst, sid, lt, active); break;
-            default: break;
-            }
-        } else if constexpr (GEN) {
+/* The compile-time code paths of the ragged kernel: (id, kind, length,
+ * variant, body).  The host stores the id of each plan's path in the plan
+ * record (fafk_rg_path), and the kernel dispatches on it with one flat
+ * switch: nested tests of kind, length and variant made the register
+ * allocator spill.  The 8192-point single pass, the path with the most
+ * registers live (112 alone), comes first: placed after the others it
+ * spilled 28-36 bytes per lane in the combined kernel, first it does not. */
+#define RG_PATHS(X)                                                          \
+    X(0, FAFD_KIND_CT, 8192, 0, (rg_rs8))                                    \
+    X(1, FAFD_KIND_CT, 256, 0, (rg_ct<256>))                                 \
+    X(2, FAFD_KIND_CT, 512, 0, (rg_ct<512>))                                 \
+    X(3, FAFD_KIND_CT, 1024, 0, (rg_ct<1024>))                               \
+    X(4, FAFD_KIND_CT, 2048, 0, (rg_ct<2048>))                               \
+    X(5, FAFD_KIND_CT, 4096, 0, (rg_ct<4096>))                               \
+    X(6, FAFD_KIND_CT, 60, 0, (rg_mix<60, 0>))                               \
+    X(7, FAFD_KIND_CT, 60, 1, (rg_mix<60, 1>))                               \
+    X(8, FAFD_KIND_CT, 360, 0, (rg_mix<360, 0>))                             \
+    X(9, FAFD_KIND_CT, 360, 1, (rg_mix<360, 1>))                             \
+    X(10, FAFD_KIND_CT, 1000, 0, (rg_mix<1000, 0>))                          \
+    X(11, FAFD_KIND_CT, 1000, 1, (rg_mix<1000, 1>))                          \
+    X(12, FAFD_KIND_CT, 3000, 0, (rg_mix<3000, 0>))                          \
+    X(13, FAFD_KIND_CT, 3000, 1, (rg_mix<3000, 1>))                          \
+    X(14, FAFD_KIND_CT, 3000, 2, (rg_mix<3000, 2>))                          \
+    X(15, FAFD_KIND_CT, 4800, 0, (rg_mix<4800, 0>))                          \
+    X(16, FAFD_KIND_CT, 4800, 1, (rg_mix<4800, 1>))                          \
+    X(17, FAFD_KIND_RADER_CT, 11, 0, (rg_rader<11, 0>))                      \
+    X(18, FAFD_KIND_RADER_CT, 11, 1, (rg_rader<11, 1>))                      \
+    X(19, FAFD_KIND_RADER_CT, 17, 0, (rg_rader<17, 0>))                      \
+    X(20, FAFD_KIND_RADER_CT, 31, 0, (rg_rader<31, 0>))                      \
+    X(21, FAFD_KIND_RADER_CT, 31, 1, (rg_rader<31, 1>))                      \
+    X(22, FAFD_KIND_RADER_CT, 97, 0, (rg_rader<97, 0>))                      \
+    X(23, FAFD_KIND_RADER_CT, 97, 1, (rg_rader<97, 1>))                      \
+    X(24, FAFD_KIND_RADER_CT, 1009, 0, (rg_rader<1009, 0>))                  \
+    X(25, FAFD_KIND_RADER_CT, 1009, 1, (rg_rader<1009, 1>))
+
+/* One unit: up to spb signals of one plan.  The switch is uniform across
+ * the workgroup, so the barriers inside every path stay uniform. */
+template <bool GEN>
+FAF_DEV void rg_unit(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst, int sid,
+                     int lt, int active) {
+    switch (pl->path) {
+#define RG_CASE(id, kind, n, v, body) \
+    case id: body(pl, src, dst, sid, lt, active); break;
+        RG_PATHS(RG_CASE)
+#undef RG_CASE
+    default:
+        if constexpr (GEN) {
             if (pl->kind == FAFD_KIND_STOCKHAM) {
                 Lds lds{faf_smem + sid * pl->fwd.plane};
                 fft_run<false>(pl->fwd, src, lds, dst, lt, active);
@@ -174,48 +173,118 @@ __global__ __launch_bounds__(MT, GEN ? 1 : 4) void kg_ragged(const fafd_rplan *p
                 const int per = pl->fwd.plane + lpad(pl->rader.p) + 1;
                 Lds xa{faf_smem + sid * per};
                 Lds aa{faf_smem + sid * per + lpad(pl->rader.p) + 1};
-                rader_body(pl->fwd, pl->inv, pl->rader, src, dst, xa, aa, lt, T, active);
-            } else {
+                rader_body(pl->fwd, pl->inv, pl->rader, src, dst, xa, aa, lt, pl->threads,
+                           active);
+            } else if (pl->kind == FAFD_KIND_BLUESTEIN) {
                 Lds aa{faf_smem + sid * pl->fwd.plane};
                 blue_body(pl->fwd, pl->inv, pl->blue, src, dst, aa, lt, active);
             }
         }
-        __syncthreads();
+        break;
     }
 }
 
+/* the code path of (kind, length, variant), or -1 if the kernel has none */
+extern "C" int fafk_rg_path(int kind, int n, int v) {
+#define RG_FIND(id, k, len, var, body) \
+    if (kind == k && n == len && v == var) return id;
+    RG_PATHS(RG_FIND)
+#undef RG_FIND
+    return -1;
+}
+
+/* Two launch shapes over the same unit body.
+ * PERSIST false: one workgroup per unit (grid = units, unit = blockIdx);
+ * the hardware dispatcher hands out the cost-ordered units in order.
+ * PERSIST true: the grid is the workgroups resident at once; workgroup b
+ * starts on unit b and then takes the next from counter[0] (set to the grid
+ * size) until the queue is empty; the last workgroup to leave resets
+ * counter[0] to the grid size and counter[1] to 0 for the next launch.
+ * uoff[i] is the offset of the signal in slot i of the unit list. */
+template <bool GEN>
+FAF_DEV void rg_run_unit(const fafd_rplan *plans, const fafd_unit *units, int u, const long *uoff,
+                         const float *in_re, const float *in_im, float *out_re, float *out_im,
+                         int tid) {
+    const fafd_unit un = units[u];
+    const fafd_rplan *pl = plans + un.plan;
+    const int T = pl->threads;
+    const int sid = tid / T, lt = tid - sid * T;
+    const int active = sid < un.count;
+    const long off = uoff[un.first + (active ? sid : 0)];
+    GSplitIn src{in_re + off, in_im + off};
+    GSplitOut dst{out_re + off, out_im + off, 1.0f};
+    rg_unit<GEN>(pl, src, dst, sid, lt, active);
+}
+
+template <int MT, bool GEN, bool PERSIST>
+__global__ __launch_bounds__(MT, GEN ? 1 : 4) void kg_ragged(const fafd_rplan *plans, const fafd_unit *units,
+                                                int nunits, const long *uoff,
+                                                const float *in_re, const float *in_im,
+                                                float *out_re, float *out_im,
+                                                unsigned *counter) {
+    const int tid = (int)threadIdx.x;
+    if constexpr (!PERSIST) {
+        rg_run_unit<GEN>(plans, units, (int)blockIdx.x, uoff, in_re, in_im, out_re, out_im, tid);
+    } else {
+        __shared__ int s_unit;
+        int u = (int)blockIdx.x;
+        while (u < nunits) {
+            rg_run_unit<GEN>(plans, units, u, uoff, in_re, in_im, out_re, out_im, tid);
+            __syncthreads();
+            if (tid == 0) s_unit = (int)atomicAdd(counter, 1u);
+            __syncthreads();
+            u = s_unit;
+        }
+        if (tid == 0) {
+            __threadfence();
+            if (atomicAdd(counter + 1, 1u) == gridDim.x - 1) {
+                atomicExch(counter, gridDim.x);
+                atomicExch(counter + 1, 0u);
+            }
+        }
+    }
+}
+
+#define RG_KERNELS(X) X(true, false) X(false, false) X(true, true) X(false, true)
+
 extern "C" int fafk_rg_init(void) {
-    (void)hipFuncSetAttribute((const void *)kg_ragged<512, true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
+#define RG_ATTR(G, P)                                                                  \
+    (void)hipFuncSetAttribute((const void *)kg_ragged<512, G, P>,                      \
+                              hipFuncAttributeMaxDynamicSharedMemorySize,              \
                               FAFD_MAX_LDS_BYTES - 64);
-    (void)hipFuncSetAttribute((const void *)kg_ragged<512, false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              FAFD_MAX_LDS_BYTES - 64);
+    RG_KERNELS(RG_ATTR)
+#undef RG_ATTR
     (void)hipGetLastError();
     return 0;
 }
 
-extern "C" int fafk_ragged(const void *plans, const void *units, int nunits,
-                           const int32_t *order, const long *offsets, const float *in_re,
-                           const float *in_im, float *out_re, float *out_im,
-                           unsigned *counter, int grid, int block, size_t lds, int generic,
-                           void *stream) {
-    hipLaunchKernelGGL(kg_fill_u32, dim3(1), dim3(64), 0, S(stream), counter, 1, 0u);
+/* counter[0] = grid, counter[1] = 0: the state kg_ragged expects (and
+ * leaves behind) */
+extern "C" int fafk_ragged_counter_init(unsigned *counter, int grid, void *stream) {
+    hipLaunchKernelGGL(kg_fill_u32, dim3(1), dim3(64), 0, S(stream), counter, 1, (unsigned)grid);
+    hipLaunchKernelGGL(kg_fill_u32, dim3(1), dim3(64), 0, S(stream), counter + 1, 1, 0u);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return rg_fail("kg_fill_u32", e);
+    return e == hipSuccess ? 0 : rg_fail("kg_fill_u32", e);
+}
+
+/* grid = nunits when not persistent */
+extern "C" int fafk_ragged(const void *plans, const void *units, int nunits, const long *uoff,
+                           const float *in_re, const float *in_im, float *out_re,
+                           float *out_im, unsigned *counter, int grid, int block, size_t lds,
+                           int generic, int persist, void *stream) {
     if (block > 512) {               /* the host plans every unit for 512 threads */
         snprintf(g_rg_err, sizeof g_rg_err, "ragged launch of %d threads per workgroup", block);
         return -1;
     }
-    if (generic)
-        hipLaunchKernelGGL((kg_ragged<512, true>), dim3(grid), dim3(block), lds, S(stream),
-                           (const fafd_rplan *)plans, (const fafd_unit *)units, nunits, order,
-                           offsets, in_re, in_im, out_re, out_im, counter);
-    else
-        hipLaunchKernelGGL((kg_ragged<512, false>), dim3(grid), dim3(block), lds, S(stream),
-                           (const fafd_rplan *)plans, (const fafd_unit *)units, nunits, order,
-                           offsets, in_re, in_im, out_re, out_im, counter);
-    e = hipGetLastError();
+    if (!persist) grid = nunits;
+#define RG_LAUNCH(G, P)                                                                \
+    if (generic == G && (persist != 0) == P)                                           \
+        hipLaunchKernelGGL((kg_ragged<512, G, P>), dim3(grid), dim3(block), lds, S(stream), \
+                           (const fafd_rplan *)plans, (const fafd_unit *)units, nunits,  \
+                           uoff, in_re, in_im, out_re, out_im, counter);
+    RG_KERNELS(RG_LAUNCH)
+#undef RG_LAUNCH
+    hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : rg_fail("kg_ragged", e);
 }
 
@@ -225,30 +294,36 @@ extern "C" const char *fafk_rg_last_error(void) { return g_rg_err; }
  * is not the plan's plane, else 0 */
 extern "C" int fafk_rg_ct_slots(int n) { return n == 8192 ? rs8::LDS_BYTES / 8 : 0; }
 
-/* threads per signal of the Rader path for prime p with compile-time inner
- * passes, 0 if p has none */
-extern "C" int fafk_rg_rader_threads(int p) {
-    switch (p) {
-    case 11: return RaderSeq<11>::T;
-    case 17: return RaderSeq<17>::T;
-    case 31: return RaderSeq<31>::T;
-    case 97: return RaderSeq<97>::T;
-    case 1009: return RaderSeq<1009>::T;
-    default: return 0;
-    }
+/* Compile-time pass sequences of the ragged launch.  For length n
+ * (Rader: prime n, the sequence of its n - 1 inner points) and variant v:
+ * the threads per signal, or 0 if there is no such sequence; with order
+ * non-NULL also its radices (returned in *nr). */
+template <class Seq, int T> static int seq_out(int *order, int *nr) {
+    if (order && nr) *nr = radices_out(typename Seq::type{}, order);
+    return T;
+}
+#define RG_SEQ(S, N, V)                                                      \
+    if (n == N && v == V) return seq_out<S<N, V>, S<N, V>::T>(order, nr);
+extern "C" int fafk_rg_rader_seq(int n, int v, int *order, int *nr) {
+    RG_SEQ(RaderSeq, 11, 0) RG_SEQ(RaderSeq, 17, 0) RG_SEQ(RaderSeq, 31, 0)
+    RG_SEQ(RaderSeq, 97, 0) RG_SEQ(RaderSeq, 1009, 0)
+    RG_SEQ(RaderSeq, 11, 1) RG_SEQ(RaderSeq, 31, 1) RG_SEQ(RaderSeq, 97, 1)
+    RG_SEQ(RaderSeq, 1009, 1)
+    return 0;
 }
 
-/* threads per signal of the compile-time passes of length n in the ragged
- * launch, 0 if n has none */
-extern "C" int fafk_rg_ct_threads(int n) {
-    switch (n) {
-    case 256: case 512: case 1024: case 2048: case 4096: return n / 16;
-    case 8192: return rs8::T;
-    case 60: return MixSeq<60>::T;
-    case 360: return MixSeq<360>::T;
-    case 1000: return MixSeq<1000>::T;
-    case 3000: return MixSeq<3000>::T;
-    case 4800: return MixSeq<4800>::T;
-    default: return 0;
+extern "C" int fafk_rg_ct_seq(int n, int v, int *order, int *nr) {
+    if (v == 0) {
+        switch (n) {
+        case 256: case 512: case 1024: case 2048: case 4096: return n / 16;
+        case 8192: return rs8::T;
+        default: break;
+        }
     }
+    RG_SEQ(MixSeq, 60, 0) RG_SEQ(MixSeq, 360, 0) RG_SEQ(MixSeq, 1000, 0)
+    RG_SEQ(MixSeq, 3000, 0) RG_SEQ(MixSeq, 4800, 0)
+    RG_SEQ(MixSeq, 60, 1) RG_SEQ(MixSeq, 360, 1) RG_SEQ(MixSeq, 1000, 1)
+    RG_SEQ(MixSeq, 3000, 1) RG_SEQ(MixSeq, 4800, 1) RG_SEQ(MixSeq, 3000, 2)
+    return 0;
 }
+#undef RG_SEQ

@@ -1,13 +1,15 @@
 /*
- * faf_ragged.c -- many C2C signals of uneven lengths in one persistent
- * launch (C11 host side), and the autotuner of that launch.
+ * faf_ragged.c -- many C2C signals of uneven lengths in one launch (C11
+ * host side), and the autotuner of that launch.
  *
  * Signals are bucketed by length; each bucket has one device plan
  * (Stockham, Rader or Bluestein) in a device-resident plan table.  Work is
  * cut into units of up to `spb` same-length signals (one workgroup's worth)
- * and the units are ordered largest-first; persistent workgroups pull units
- * from an atomic counter until the queue is empty, so lengths from 11 to
- * 8192 share one launch without wave divergence inside a unit.
+ * and the units are ordered largest-first, so lengths from 11 to 8192 share
+ * one launch without wave divergence inside a unit.  The launch either
+ * gives every unit a workgroup of its own (dispatched in that order), or
+ * runs persistent workgroups that pull units from an atomic counter until
+ * the queue is empty; the autotuner times both.
  *
  * How each length runs -- compile-time passes or a runtime radix plan (and
  * which radix order), and how many signals a unit carries -- is a per-length
@@ -25,7 +27,7 @@
 #define RAGGED_BLOCK 512
 #define RAGGED_LDS_CAP (96 * 1024)
 /* with compile-time paths only, two workgroups share a CU's 160 KB */
-#define RAGGED_LDS_CAP_CT (79 * 1024)
+#define RAGGED_LDS_CAP_CT (80 * 1024 - 256)
 
 /* how one length runs in the launch */
 typedef struct rg_choice {
@@ -33,6 +35,7 @@ typedef struct rg_choice {
     int norder;               /* runtime Stockham: radix order (0 = default) */
     int order[FAFD_MAX_STAGES];
     int spb;                  /* signals per unit (0 = as many as fit) */
+    int variant;              /* compile-time kinds: pass sequence (-1: the default) */
 } rg_choice;
 
 struct faf_ragged {
@@ -41,9 +44,10 @@ struct faf_ragged {
     uint32_t flags;
     int nplans;
     fafh_plan **hp;           /* host plans own the device tables */
-    void *d_plans, *d_units, *d_order, *d_off, *d_counter;
+    void *d_plans, *d_units, *d_uoff, *d_counter;
     int nunits;
     int generic;              /* some length needs the runtime paths */
+    int persist;              /* 1: persistent workgroups and a queue; 0: one per unit */
     int grid;
     size_t lds;
 };
@@ -98,9 +102,9 @@ static void ragged_free_device(faf_ragged *r) {
     free(r->hp);
     r->hp = NULL;
     r->nplans = 0;
-    fafk_free(r->d_plans); fafk_free(r->d_units); fafk_free(r->d_order);
-    fafk_free(r->d_off); fafk_free(r->d_counter);
-    r->d_plans = r->d_units = r->d_order = r->d_off = r->d_counter = NULL;
+    fafk_free(r->d_plans); fafk_free(r->d_units); fafk_free(r->d_uoff);
+    fafk_free(r->d_counter);
+    r->d_plans = r->d_units = r->d_uoff = r->d_counter = NULL;
 }
 
 void faf_ragged_destroy(faf_ragged *r) {
@@ -116,27 +120,55 @@ size_t faf_ragged_total(const faf_ragged *r) { return r ? r->total : 0; }
  * planner's default).  Returns 0 and fills d / *per (LDS float2 per signal),
  * or -1 with an error.  A choice that the length cannot take (compile-time
  * passes it has none of) fails, so the autotuner skips it. */
+/* The pass sequence a compile-time kind of length n (rader: the n - 1
+ * inner points of prime n) runs when no choice names one: the composite
+ * radices (variant 1) where the length has them, else the planner's order */
+static int rg_default_variant(size_t n, int rader) {
+    if (n > 8192) return 0;
+    return (rader ? fafk_rg_rader_seq((int)n, 1, NULL, NULL)
+                  : fafk_rg_ct_seq((int)n, 1, NULL, NULL)) > 0 ? 1 : 0;
+}
+
+/* records radices ord[0..nr) as the stage list of P (for reporting: the
+ * compile-time kinds take their passes from the kernel's own sequence) */
+static void set_order(fafd_fft *P, const int *ord, int nr) {
+    int32_t ns = 1;
+    P->nstages = nr;
+    for (int s = 0; s < nr; ++s) {
+        P->st[s].radix = ord[s];
+        P->st[s].ns = ns;
+        P->st[s].tw_stride = P->n / (ns * ord[s]);
+        ns *= ord[s];
+    }
+}
+
 static int plan_length(fafh_plan *p, size_t n, uint32_t flags, const rg_choice *c,
                        fafd_rplan *d, size_t *per) {
     if (fafi_plan_c2c(p, n, 0, (flags & FAF_FLAG_BLUESTEIN) != 0)) return -1;
     memset(d, 0, sizeof *d);
     d->len = (int32_t)n;
+    d->path = -1;
     const int want = c ? c->kind : -1;
     if (p->kind == PK_C2C) {
-        /* the planner's own radix order: the compile-time passes of the
-         * 5-smooth lengths follow it (a plan-level autotune may have changed
-         * p->a, so it is rebuilt here) */
+        /* the planner's own radix order: variant 0 of the compile-time passes
+         * of the 5-smooth lengths follows it (a plan-level autotune may have
+         * changed p->a, so it is rebuilt here) */
         fafd_fft def;
         if (fafi_make_fft(n, 0, 1, NULL, 0, &def)) return -1;
-        const int ct_t = fafi_ct_enabled() && n <= 8192 ? fafk_rg_ct_threads((int)n) : 0;
+        const int var = want == FAFD_KIND_CT && c->variant >= 0 ? c->variant : rg_default_variant(n, 0);
+        int ord[FAFD_MAX_STAGES], nr = 0;
+        const int ct_t = fafi_ct_enabled() && n <= 8192 ? fafk_rg_ct_seq((int)n, var, ord, &nr) : 0;
         const int pow2 = !(n & (n - 1));
-        const int ct_ok = ct_t > 0 && (pow2 || ct_t == def.threads);
+        const int ct_ok = ct_t > 0 && (pow2 || var > 0 || ct_t == def.threads);
         int kind = want >= 0 ? want : (ct_ok ? FAFD_KIND_CT : FAFD_KIND_STOCKHAM);
         if (kind == FAFD_KIND_CT) {
             if (!ct_ok) { fafi_err("ragged: length %zu has no compile-time passes", n); return -1; }
             d->kind = FAFD_KIND_CT;
+            d->variant = var;
+            d->path = fafk_rg_path(FAFD_KIND_CT, (int)n, var);
             d->fwd = def;
             d->fwd.threads = ct_t;
+            if (nr > 0) set_order(&d->fwd, ord, nr);
             *per = (size_t)def.plane;
             if (fafk_rg_ct_slots((int)n) > 0) *per = (size_t)fafk_rg_ct_slots((int)n);
         } else if (kind == FAFD_KIND_STOCKHAM) {
@@ -152,7 +184,9 @@ static int plan_length(fafh_plan *p, size_t n, uint32_t flags, const rg_choice *
             return -1;
         }
     } else if (p->kind == PK_RADER) {
-        const int rt = fafi_ct_enabled() ? fafk_rg_rader_threads((int)n) : 0;
+        const int var = want == FAFD_KIND_RADER_CT && c->variant >= 0 ? c->variant : rg_default_variant(n, 1);
+        int ord[FAFD_MAX_STAGES], nr = 0;
+        const int rt = fafi_ct_enabled() ? fafk_rg_rader_seq((int)n, var, ord, &nr) : 0;
         int kind = want >= 0 ? want : (rt > 0 ? FAFD_KIND_RADER_CT : FAFD_KIND_RADER);
         if (kind == FAFD_KIND_RADER_CT && rt <= 0) {
             fafi_err("ragged: prime %zu has no compile-time inner passes", n);
@@ -164,7 +198,12 @@ static int plan_length(fafh_plan *p, size_t n, uint32_t flags, const rg_choice *
         }
         d->kind = kind;
         d->fwd = p->a; d->inv = p->b; d->rader = p->rader;
-        if (kind == FAFD_KIND_RADER_CT) d->fwd.threads = rt;
+        if (kind == FAFD_KIND_RADER_CT) {
+            d->variant = var;
+            d->path = fafk_rg_path(FAFD_KIND_RADER_CT, (int)n, var);
+            d->fwd.threads = rt;
+            if (nr > 0) set_order(&d->fwd, ord, nr);
+        }
         *per = (size_t)p->a.plane + n + (n >> 4) + 1;
     } else if (p->kind == PK_BLUE) {
         if (want >= 0 && want != FAFD_KIND_BLUESTEIN) {
@@ -179,6 +218,10 @@ static int plan_length(fafh_plan *p, size_t n, uint32_t flags, const rg_choice *
         return -1;
     }
     d->threads = d->fwd.threads;
+    if ((d->kind == FAFD_KIND_CT || d->kind == FAFD_KIND_RADER_CT) && d->path < 0) {
+        fafi_err("ragged: length %zu has no compiled path", n);
+        return -1;
+    }
     if (d->threads > RAGGED_BLOCK) {
         fafi_err("ragged: length %zu needs %d threads", n, d->threads);
         return -1;
@@ -196,6 +239,17 @@ static const rg_choice *choice_for(size_t n, const rg_overrides *ov, rg_choice *
             if (ov->len[i] == n) return &ov->c[i];
     if (rg_tuned_find(n, tmp)) return tmp;
     return NULL;
+}
+
+/* Workgroups of the ragged kernel a CU holds at once: 8 waves each; the
+ * lean kernel's 128 VGPRs allow 4 waves per SIMD (16 per CU), the one with
+ * the runtime paths 2; and the CU's 160 KB of LDS, in 512-byte blocks. */
+static int ragged_per_cu(int generic, size_t lds) {
+    const int by_waves = generic ? 1 : 2;
+    const size_t blk = (lds + 64 + 511) / 512 * 512;
+    const int by_lds = (int)(FAFD_MAX_LDS_BYTES / blk);
+    const int n = by_waves < by_lds ? by_waves : by_lds;
+    return n < 1 ? 1 : n;
 }
 
 /* Builds the device tables of a ragged batch into r (r->lengths/count set). */
@@ -247,7 +301,11 @@ static int ragged_build(faf_ragged *r, const rg_overrides *ov) {
             break;
         }
         d->spb = spb;
-        if ((size_t)spb * per * 8 > lds_max) lds_max = (size_t)spb * per * 8;
+        /* the 8192-point single pass exchanges through LDS in every signal
+         * slot of the workgroup, active or not, so its units reserve all */
+        const size_t slots = fafk_rg_ct_slots(d->len) > 0 && d->kind == FAFD_KIND_CT
+                                 ? (size_t)(RAGGED_BLOCK / d->threads) : (size_t)spb;
+        if (slots * per * 8 > lds_max) lds_max = slots * per * 8;
     }
     free(pers);
     free(want_spb);
@@ -300,18 +358,31 @@ static int ragged_build(faf_ragged *r, const rg_overrides *ov) {
     for (int i = 0; i < nuu; ++i) units[i] = uc[i].u;
     r->nunits = nuu;
     r->lds = lds_max;
-    long g = 4 * (long)fafi_cus();
+    /* persistent grid: the workgroups resident at once (each starts on its
+     * own unit without a queue round trip), never more than the units */
+    int per_cu = ragged_per_cu(r->generic, r->lds);
+    long g = (long)per_cu * (long)fafi_cus();
     r->grid = (int)(nuu < g ? nuu : g);
     if (r->grid < 1) r->grid = 1;
 
+    /* offset of the signal in each slot of the unit list */
+    long *uoff = malloc(count * sizeof(long));
+    for (size_t i = 0; i < count; ++i) uoff[i] = off[order[i]];
     r->d_plans = up(tab, (size_t)nu * sizeof(fafd_rplan));
     r->d_units = up(units, (size_t)nuu * sizeof(fafd_unit));
-    r->d_order = up(order, count * sizeof(int32_t));
-    r->d_off = up(off, count * sizeof(long));
+    r->d_uoff = up(uoff, count * sizeof(long));
     r->d_counter = fafk_malloc(256);
     free(uniq); free(tab); free(order); free(off); free(pid); free(nper); free(start);
-    free(fill); free(uc); free(units);
-    if (!r->d_plans || !r->d_units || !r->d_order || !r->d_off || !r->d_counter) {
+    free(fill); free(uc); free(units); free(uoff);
+    if (!r->d_plans || !r->d_units || !r->d_uoff || !r->d_counter) {
+        fafi_err("ragged: %s", fafk_last_error());
+        return -1;
+    }
+    if (fafk_ragged_counter_init(r->d_counter, r->grid, fafi_stream())) {
+        fafi_err("ragged: %s", fafk_rg_last_error());
+        return -1;
+    }
+    if (fafk_sync(fafi_stream())) {
         fafi_err("ragged: %s", fafk_last_error());
         return -1;
     }
@@ -352,9 +423,9 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
 
 static int launch(faf_ragged *r, float *out_re, float *out_im, const float *in_re,
                   const float *in_im, void *stream) {
-    int rc = fafk_ragged(r->d_plans, r->d_units, r->nunits, r->d_order, r->d_off, in_re, in_im,
+    int rc = fafk_ragged(r->d_plans, r->d_units, r->nunits, r->d_uoff, in_re, in_im,
                          out_re, out_im, r->d_counter, r->grid, RAGGED_BLOCK, r->lds,
-                         r->generic, stream);
+                         r->generic, r->persist, stream);
     if (rc) fafi_err("ragged: %s", fafk_rg_last_error());
     return rc;
 }
@@ -382,7 +453,7 @@ int faf_ragged_execute(faf_ragged *r, float *out_re, float *out_im, const float 
 /* autotuner of the ragged launch                                       */
 /* ------------------------------------------------------------------ */
 
-#define RG_MAX_CANDS 24
+#define RG_MAX_CANDS 48
 
 /* median launch time (ms) of r over 5 timed launches after 2 warm ones */
 static float time_ragged(faf_ragged *r, float *buf, void *stream) {
@@ -443,15 +514,26 @@ static int candidates(size_t n, uint32_t flags, rg_choice *out) {
         }
     }
     for (int a = 0; a < nk; ++a) {
-        const int nord = kinds[a] == FAFD_KIND_STOCKHAM ? no : 1;
+        /* runtime passes: every radix order; compile-time kinds: every pass
+         * sequence the kernel carries for this length */
+        int nord = 1;
+        if (kinds[a] == FAFD_KIND_STOCKHAM) nord = no;
+        else if (kinds[a] == FAFD_KIND_CT || kinds[a] == FAFD_KIND_RADER_CT)
+            while (nord < 4 && (kinds[a] == FAFD_KIND_CT
+                                    ? fafk_rg_ct_seq((int)n, nord, NULL, NULL)
+                                    : fafk_rg_rader_seq((int)n, nord, NULL, NULL)) > 0)
+                ++nord;
         for (int o = 0; o < nord; ++o)
             for (int s = 0; s < 3 && nc < RG_MAX_CANDS; ++s) {
                 rg_choice c;
                 memset(&c, 0, sizeof c);
                 c.kind = kinds[a];
+                c.variant = -1;
                 if (kinds[a] == FAFD_KIND_STOCKHAM) {
                     c.norder = lens[o];
                     memcpy(c.order, orders[o], (size_t)lens[o] * sizeof(int));
+                } else if (kinds[a] != FAFD_KIND_BLUESTEIN && kinds[a] != FAFD_KIND_RADER) {
+                    c.variant = o;
                 }
                 c.spb = spbs[s];          /* negative: a fraction, resolved below */
                 out[nc++] = c;
@@ -568,11 +650,15 @@ int faf_ragged_autotune(faf_ragged *r) {
         const rg_overrides *ovs[3] = {&ov_best, &ov_lean, &ov_dflt};
         const rg_choice *win = NULL;
         float twin = 1e30f;
+        int win_persist = 0;
         for (int c = 0; c < 3; ++c) {
             faf_ragged *a = ragged_new(r->lengths, r->count, r->flags, ovs[c]);
-            float ta = a ? time_ragged(a, buf, stream) : -1.0f;
+            for (int ps = 0; a && ps < 2; ++ps) {      /* one workgroup per unit, persistent */
+                a->persist = ps;
+                const float ta = time_ragged(a, buf, stream);
+                if (ta > 0.0f && ta < twin) { twin = ta; win = cand[c]; win_persist = ps; }
+            }
             faf_ragged_destroy(a);
-            if (ta > 0.0f && ta < twin) { twin = ta; win = cand[c]; }
         }
         if (!win) {
             fafi_err("ragged autotune: the batch did not run");
@@ -583,6 +669,7 @@ int faf_ragged_autotune(faf_ragged *r) {
             ragged_free_device(r);
             rg_overrides ov = {uniq, win, nu};
             if (ragged_build(r, &ov)) rc = -1;
+            r->persist = win_persist;
         }
         free(lean);
         free(dflt);
@@ -621,4 +708,67 @@ int faf_ragged_choice(const faf_ragged *r, size_t len, int *kind, int *spb, int 
     }
     fafi_err("ragged: length %zu is not in this batch", len);
     return -1;
+}
+
+int faf_ragged_set_choice(size_t len, int kind, int spb, const int *order, int norder) {
+    if (len < 2 || spb < 0 || norder < 0 || norder > FAFD_MAX_STAGES || (norder && !order)) {
+        fafi_err("ragged choice: bad argument");
+        return -1;
+    }
+    if (kind == -1) {                    /* forget the length: back to the defaults */
+        pthread_mutex_lock(&g_rg_mu);
+        for (int i = 0; i < g_rg_ntuned; ++i)
+            if (g_rg_tuned[i].len == len) {
+                g_rg_tuned[i] = g_rg_tuned[--g_rg_ntuned];
+                break;
+            }
+        pthread_mutex_unlock(&g_rg_mu);
+        return 0;
+    }
+    if (fafi_create_ready()) return -1;
+    rg_choice c;
+    memset(&c, 0, sizeof c);
+    c.kind = kind;
+    c.spb = spb;
+    c.variant = -1;
+    if (kind == FAFD_KIND_CT || kind == FAFD_KIND_RADER_CT) {
+        /* the order names one of the kernel's compiled pass sequences */
+        for (int v = 0; norder > 0 && c.variant < 0 && v < 4; ++v) {
+            int ord[FAFD_MAX_STAGES], nr = 0;
+            const int t = kind == FAFD_KIND_CT ? fafk_rg_ct_seq((int)len, v, ord, &nr)
+                                               : fafk_rg_rader_seq((int)len, v, ord, &nr);
+            if (t > 0 && nr == norder && !memcmp(ord, order, (size_t)nr * sizeof(int)))
+                c.variant = v;
+        }
+        if (norder > 0 && c.variant < 0) {
+            fafi_err("ragged choice: no compile-time pass sequence of length %zu in that order",
+                     len);
+            return -1;
+        }
+    } else if (kind == FAFD_KIND_STOCKHAM) {
+        c.norder = norder;
+        if (norder) memcpy(c.order, order, (size_t)norder * sizeof(int));
+    } else if (kind != FAFD_KIND_RADER && kind != FAFD_KIND_BLUESTEIN) {
+        fafi_err("ragged choice: unknown kind %d", kind);
+        return -1;
+    }
+    /* the length must plan that way */
+    fafh_plan *p = fafi_plan_new();
+    fafd_rplan d;
+    size_t per = 0;
+    const int rc = p ? plan_length(p, len, kind == FAFD_KIND_BLUESTEIN ? FAF_FLAG_BLUESTEIN : 0,
+                                   &c, &d, &per)
+                     : -1;
+    fafi_plan_free(p);
+    if (rc) return -1;
+    rg_tuned_store(len, &c);
+    return 0;
+}
+
+int faf_ragged_persistent(const faf_ragged *r) { return r ? r->persist : -1; }
+
+int faf_ragged_set_persistent(faf_ragged *r, int on) {
+    if (!r) { fafi_err("ragged: null plan"); return -1; }
+    r->persist = on != 0;
+    return 0;
 }

@@ -79,7 +79,7 @@ int faf_execute_batch(const faf_transform *t, faf_buffer *out, const faf_buffer 
 int faf_execute_batch_noalloc(const faf_transform *t, faf_buffer *out, const faf_buffer *in,
                               size_t count, size_t in_dist, size_t out_dist);
 
-/* ---- ragged C2C batch in one persistent launch ----
+/* ---- ragged C2C batch in one launch ----
  * Signals of different lengths are concatenated in split planes at
  * offsets = exclusive prefix sum of lengths.  Every length must be legal
  * for faf_create_fft (5-smooth, codelet, or Rader prime; others need
@@ -107,6 +107,20 @@ int faf_ragged_autotune(faf_ragged *r);
  * passes), signals per unit, and the radix order of its (inner) FFT. */
 int faf_ragged_choice(const faf_ragged *r, size_t len, int *kind, int *spb, int *order,
                       int *norder);
+/* Records for the process how length `len` runs in ragged plans created
+ * afterwards, in faf_ragged_choice's terms: kind (as above), signals per
+ * unit (0: as many as fit) and the radix order of its (inner) FFT (norder
+ * 0: the default order of that kind).  A compile-time kind accepts only
+ * the orders its kernel carries; kind -1 forgets the length's record, so
+ * it runs by the planner's defaults (and the autotuner times it again).
+ * Returns 0, or -1 with a message when the length cannot run that way. */
+int faf_ragged_set_choice(size_t len, int kind, int spb, const int *order, int norder);
+/* Launch shape of r: 0 = one workgroup per unit (the units are dispatched
+ * largest first; the default), 1 = persistent workgroups, one per resident
+ * slot, that take units from a device queue until it is empty.
+ * faf_ragged_autotune times both on the batch and keeps the faster. */
+int faf_ragged_persistent(const faf_ragged *r);
+int faf_ragged_set_persistent(faf_ragged *r, int on);
 
 /* ---- framed real STFT ----
  * faf_create_stft with cfg.layout == FAF_LAYOUT_REAL (an error in the

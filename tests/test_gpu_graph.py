@@ -216,16 +216,21 @@ def test_graph_survives_workspace_growth():
         G.destroy(t)
 
 
-def test_graph_ragged_replays():
-    """The ragged launch resets its work counter with a fill kernel, so replays
-    start from an empty queue every time."""
-    say("case: graph ragged")
+@pytest.mark.parametrize("persist", [0, 1])
+def test_graph_ragged_replays(persist):
+    """Replays of a captured ragged launch match eager runs interleaved with
+    them, in both launch shapes: the persistent kernel's last workgroup resets
+    the work counter for the next launch, so every replay starts from a full
+    queue."""
+    say(f"case: graph ragged persist={persist}")
     lib = G.L()
     rng = np.random.default_rng(77)
     lens = [int(v) for v in rng.choice([60, 360, 1000, 11, 17, 97, 1009, 4096], 300)]
     arr = (C.c_size_t * len(lens))(*lens)
     r = lib.faf_ragged_create(arr, len(lens), 0)
     assert r, err()
+    fa.check(lib.faf_ragged_set_persistent(r, persist), "set persistent")
+    assert lib.faf_ragged_persistent(r) == persist
     tot = lib.faf_ragged_total(r)
     assert tot == sum(lens)
     off = np.concatenate([[0], np.cumsum(lens)])

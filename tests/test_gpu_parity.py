@@ -570,6 +570,42 @@ def test_c3_ragged_persistent_launch():
         lib.faf_ragged_destroy(r)
 
 
+@pytest.mark.parametrize("lengths,flags", [
+    (C3_LENGTHS, 0),
+    ([4050, 128, 13, 19, 2000, 23, 97, 8192, 24, 1009, 60, 4096], 1)])
+def test_ragged_launch_shapes_agree(lengths, flags):
+    """One workgroup per unit and persistent workgroups run the same unit code:
+    every signal has the same bits in both shapes (lean kernel, and the one
+    with the runtime paths), over several launches of each."""
+    say(f"case: ragged launch shapes {lengths[:3]}...")
+    lib = G.L()
+    rng = np.random.default_rng(5)
+    lens = [int(v) for v in rng.choice(lengths, 1500)]
+    arr = (C.c_size_t * len(lens))(*lens)
+    r = lib.faf_ragged_create(arr, len(lens), fa.FLAG_BLUESTEIN if flags else 0)
+    assert r, lib.faf_get_error()
+    try:
+        tot = lib.faf_ragged_total(r)
+        x = (rng.uniform(-1, 1, tot) + 1j * rng.uniform(-1, 1, tot)).astype(np.complex64)
+        dxr, dxi = G.dev(x.real.copy()), G.dev(x.imag.copy())
+        outs = []
+        for persist in (0, 1, 0, 1):
+            fa.check(lib.faf_ragged_set_persistent(r, persist), "set persistent")
+            yr, yi = G.empty(tot), G.empty(tot)
+            for _ in range(3):
+                fa.check(lib.faf_ragged_execute(r, yr.data_ptr(), yi.data_ptr(),
+                                                dxr.data_ptr(), dxi.data_ptr()), "ragged")
+            outs.append(G.host(yr) + 1j * G.host(yi))
+        for o in outs[1:]:
+            assert np.array_equal(o, outs[0])
+        off = np.concatenate([[0], np.cumsum(lens)])
+        for i in list(range(0, len(lens), 13)) + [len(lens) - 1]:
+            a, e = off[i], off[i + 1]
+            assert rel_l2(outs[1][a:e], O.fft64(x[a:e].astype(np.complex128))[0]) < tol(lens[i])
+    finally:
+        lib.faf_ragged_destroy(r)
+
+
 def test_ragged_runtime_paths():
     """Lengths without compile-time passes (and a Bluestein length) put the
     launch on the kernel that carries the runtime Stockham, Rader and
@@ -1188,3 +1224,69 @@ def test_rader_bluestein_beyond_one_workgroup(n, blue):
     finally:
         G.destroy(t)
         G.destroy(ti)
+
+
+# compile-time pass sequences the ragged kernel carries per length: the
+# planner's order first, then the composite-radix ones (prime-factor codelets)
+RAGGED_CT_SEQS = {
+    4800: [[16, 4, 3, 5, 5], [12, 20, 20]],
+    3000: [[8, 3, 5, 5, 5], [10, 20, 15], [30, 10, 10]],
+    1000: [[8, 5, 5, 5], [10, 10, 10]],
+    360: [[8, 3, 3, 5], [24, 15]],
+    60: [[4, 3, 5], [15, 4]],
+}
+RAGGED_RADER_SEQS = {
+    11: [[2, 5], [10]],
+    17: [[16]],
+    31: [[2, 3, 5], [6, 5]],
+    97: [[16, 2, 3], [8, 12]],
+    1009: [[16, 3, 3, 7], [12, 12, 7]],
+}
+
+
+def test_ragged_every_compiled_sequence():
+    """Every compile-time pass sequence of every C3 length, forced through
+    faf_ragged_set_choice, in a batch of 300 signals of that length plus a
+    mixed batch: every checked signal matches FP64, and faf_ragged_choice
+    reports the sequence that ran.  The records are forgotten afterwards."""
+    lib = G.L()
+    rng = np.random.default_rng(31)
+    cases = [(n, 3, o) for n, seqs in RAGGED_CT_SEQS.items() for o in seqs]
+    cases += [(p, 4, o) for p, seqs in RAGGED_RADER_SEQS.items() for o in seqs]
+    try:
+        for n, kind, order in cases:
+            say(f"case: ragged sequence n={n} kind={kind} order={order}")
+            arr_o = (C.c_int * len(order))(*order)
+            fa.check(lib.faf_ragged_set_choice(n, kind, 0, arr_o, len(order)), "set choice")
+            lens = [n] * 300 + [int(v) for v in rng.choice([8192, 17, 4096], 40)]
+            rng.shuffle(lens)
+            arr = (C.c_size_t * len(lens))(*lens)
+            r = lib.faf_ragged_create(arr, len(lens), 0)
+            assert r, lib.faf_get_error()
+            try:
+                k_, spb_, no_ = C.c_int(), C.c_int(), C.c_int()
+                ord_ = (C.c_int * 16)()
+                fa.check(lib.faf_ragged_choice(r, n, C.byref(k_), C.byref(spb_), ord_,
+                                               C.byref(no_)), "choice")
+                assert k_.value == kind and list(ord_[: no_.value]) == order
+                off = np.concatenate([[0], np.cumsum(lens)])
+                x = (rng.uniform(-1, 1, int(off[-1])) +
+                     1j * rng.uniform(-1, 1, int(off[-1]))).astype(np.complex64)
+                dxr, dxi = G.dev(x.real.copy()), G.dev(x.imag.copy())
+                yr, yi = G.empty(int(off[-1])), G.empty(int(off[-1]))
+                fa.check(lib.faf_ragged_execute(r, yr.data_ptr(), yi.data_ptr(),
+                                                dxr.data_ptr(), dxi.data_ptr()), "ragged")
+                y = G.host(yr) + 1j * G.host(yi)
+                for i in list(range(0, len(lens), 11)) + [len(lens) - 1]:
+                    a, e = off[i], off[i + 1]
+                    truth = O.fft64(x[a:e].astype(np.complex128))[0]
+                    assert rel_l2(y[a:e], truth) < tol(lens[i]), (n, order, i, lens[i])
+            finally:
+                lib.faf_ragged_destroy(r)
+        # an order the kernel does not carry is refused with a message
+        bad = (C.c_int * 3)(20, 12, 20)
+        assert lib.faf_ragged_set_choice(4800, 3, 0, bad, 3) == -1
+        assert b"order" in lib.faf_get_error()
+    finally:
+        for n, kind, _ in cases:
+            lib.faf_ragged_set_choice(n, -1, 0, None, 0)
