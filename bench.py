@@ -423,14 +423,37 @@ def run_extras(lib, fa, torch, np, timed, extra, only, err):
         err["c1_fft_4096x256"] = rel_l2_max(
             np, yr.double().cpu().numpy() + 1j * yi.double().cpu().numpy(), np.fft.fft(xc, axis=-1))
 
-        # the same forward+inverse captured once into a HIP graph and replayed
-        fa.check(lib.faf_graph_begin(), "graph begin")
-        rcs = (lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0),
-               lib.faf_execute_batch(ti, C.byref(bz), C.byref(bo), b, 0, 0))
-        g = lib.faf_graph_end()
-        if rcs != (0, 0) or not g:
-            raise RuntimeError("graph capture: " + lib.faf_get_error().decode())
+        # the same forward+inverse captured into HIP graphs and replayed: one
+        # pair per graph, and a sequence of 16 pairs per graph (a graph
+        # launch has a fixed host cost of ~15 us, which the sequence shares)
+        def capture(pairs):
+            fa.check(lib.faf_graph_begin(), "graph begin")
+            rcs = [(lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0),
+                    lib.faf_execute_batch(ti, C.byref(bz), C.byref(bo), b, 0, 0))
+                   for _ in range(pairs)]
+            g_ = lib.faf_graph_end()
+            if any(rc != (0, 0) for rc in rcs) or not g_:
+                raise RuntimeError("graph capture: " + lib.faf_get_error().decode())
+            return g_
+
+        def host_us(fn, reps):
+            """host time per call of fn, issued back to back"""
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            dt = time.perf_counter() - t0
+            torch.cuda.synchronize()
+            return dt / reps * 1e6
+
+        g = capture(1)
+        g16 = capture(16)
         ms_g = timed(lambda: fa.check(lib.faf_graph_launch(g), "graph"), 200, 50)
+        ms_g16 = timed(lambda: fa.check(lib.faf_graph_launch(g16), "graph"), 20, 5) / 16
+        pair = lambda: (lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0),
+                        lib.faf_execute_batch(ti, C.byref(bz), C.byref(bo), b, 0, 0))
+        host_e = host_us(pair, 200)
+        host_g16 = host_us(lambda: lib.faf_graph_launch(g16), 20) / 16
 
         def latency(fn, reps=300):
             """host wall time of one call + synchronize (what a caller waits)"""
@@ -443,11 +466,13 @@ def run_extras(lib, fa, torch, np, timed, extra, only, err):
                 torch.cuda.synchronize()
             return (time.perf_counter() - t0) / reps * 1e6
 
-        lat_e = latency(lambda: (lib.faf_execute_batch(t, C.byref(bo), C.byref(bi), b, 0, 0),
-                                 lib.faf_execute_batch(ti, C.byref(bz), C.byref(bo), b, 0, 0)))
+        lat_e = latency(pair)
         lat_g = latency(lambda: lib.faf_graph_launch(g))
         extra["c1_graph_fwd_inv_4096x256_us"] = round(ms_g * 1e3, 2)
+        extra["c1_graph16_fwd_inv_4096x256_us_per_pair"] = round(ms_g16 * 1e3, 2)
+        extra["c1_host_us_per_pair"] = {"eager": round(host_e, 2), "graph16": round(host_g16, 2)}
         extra["c1_fwd_inv_call_latency_us"] = {"eager": round(lat_e, 1), "graph": round(lat_g, 1)}
+        lib.faf_graph_destroy(g16)
         lib.faf_graph_destroy(g)
         lib.faf_destroy_transform(ti)
         lib.faf_destroy_transform(t)
