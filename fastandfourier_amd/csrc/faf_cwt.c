@@ -10,6 +10,7 @@
  */
 #define _GNU_SOURCE
 #include "faf_internal.h"
+#include <pthread.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -70,6 +71,21 @@ static double meyer_nu(double x) {
     return x * x * x * x * (35.0 - 84.0 * x + 70.0 * x * x - 20.0 * x * x * x);
 }
 
+/* peak normalisation of the generalized Morse prototype: lp = b log(wp) -
+ * wp^g with wp = (b/g)^(1/g), the log of its value at the peak.  The last
+ * pair (b, g) is kept per thread, since a bank evaluates one prototype
+ * hundreds of thousands of times (same bits as computing it every call). */
+static double morse_lp(double b, double g) {
+    static _Thread_local double kb = -1.0, kg = -1.0, klp;
+    if (b != kb || g != kg) {
+        const double wp = pow(b / g, 1.0 / g);
+        klp = b * log(wp) - pow(wp, g);
+        kb = b;
+        kg = g;
+    }
+    return klp;
+}
+
 static double proto(double w, const faf_cwt_config *c) {
     if (w <= 0.0) return 0.0;
     switch (c->wavelet) {
@@ -80,9 +96,7 @@ static double proto(double w, const faf_cwt_config *c) {
     case FAF_CWT_WAVELET_MORSE: {
         /* peak-normalised generalized Morse, evaluated in the log domain */
         double g = c->morse_gamma, b = c->morse_beta;
-        double wp = pow(b / g, 1.0 / g);
-        double lp = b * log(wp) - pow(wp, g);
-        return exp(b * log(w) - pow(w, g) - lp);
+        return exp(b * log(w) - pow(w, g) - morse_lp(b, g));
     }
     case FAF_CWT_WAVELET_BUMP: {
         double u = (w - c->bump_center) / c->bump_width;
@@ -131,6 +145,86 @@ static double admissibility(const faf_cwt_config *c, double xi0) {
     return sum * dw;
 }
 
+/* The two Riemann sums above depend on the prototype alone (family and
+ * shape parameters), not on n or the scales, and take 1.3M prototype
+ * evaluations: a small process-wide table keeps them per prototype. */
+typedef struct {
+    int wavelet, center_kind;
+    double mu, gamma, beta, bc, bw;
+    double xi0, adm;
+} proto_sums;
+static proto_sums g_sums[32];
+static int g_nsums;
+static pthread_mutex_t g_sums_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static int sums_match(const proto_sums *e, const faf_cwt_config *c) {
+    return e->wavelet == (int)c->wavelet && e->center_kind == (int)c->center_kind &&
+           e->mu == c->morlet_mu && e->gamma == c->morse_gamma && e->beta == c->morse_beta &&
+           e->bc == c->bump_center && e->bw == c->bump_width;
+}
+
+/* xi0 (the energy centroid or the peak, as c asks) and the admissibility
+ * constant of c's prototype */
+static void prototype_sums(const faf_cwt_config *c, double *xi0, double *adm) {
+    pthread_mutex_lock(&g_sums_mu);
+    for (int i = 0; i < g_nsums; ++i)
+        if (sums_match(&g_sums[i], c)) {
+            *xi0 = g_sums[i].xi0;
+            *adm = g_sums[i].adm;
+            pthread_mutex_unlock(&g_sums_mu);
+            return;
+        }
+    pthread_mutex_unlock(&g_sums_mu);
+    const double x = c->center_kind == FAF_CWT_CENTER_ENERGY ? energy_omega(c) : peak_omega(c);
+    const double a = x > 0.0 && x == x ? admissibility(c, x) : 0.0;
+    pthread_mutex_lock(&g_sums_mu);
+    proto_sums e = {(int)c->wavelet, (int)c->center_kind, c->morlet_mu, c->morse_gamma,
+                    c->morse_beta, c->bump_center, c->bump_width, x, a};
+    g_sums[g_nsums < 32 ? g_nsums++ : 31] = e;
+    pthread_mutex_unlock(&g_sums_mu);
+    *xi0 = x;
+    *adm = a;
+}
+
+/* rows[j][k] = proto(scales[j] xi[k]) with DC and Nyquist zeroed, for
+ * j < J.  Rows are independent, so a large bank fills them on up to eight
+ * threads (each row is computed by the same code either way). */
+typedef struct {
+    const faf_cwt_config *c;
+    const double *scales, *xi;
+    size_t J, nb, stride, j0, step;
+    double *rows;
+} proto_job;
+
+static void *proto_rows_run(void *arg) {
+    const proto_job *q = arg;
+    for (size_t j = q->j0; j < q->J; j += q->step) {
+        double *row = q->rows + j * q->stride;
+        for (size_t k = 0; k < q->nb; ++k) row[k] = proto(q->scales[j] * q->xi[k], q->c);
+        row[0] = 0.0;
+        if (q->nb > 1) row[q->nb - 1] = 0.0;
+    }
+    return NULL;
+}
+
+static void proto_rows(const faf_cwt_config *c, const double *scales, size_t J,
+                       const double *xi, size_t nb, double *rows, size_t stride) {
+    enum { MAXT = 8 };
+    size_t nt = J * nb >= 65536 ? (J < MAXT ? J : MAXT) : 1;
+    proto_job jobs[MAXT];
+    pthread_t th[MAXT];
+    int started[MAXT] = {0};
+    for (size_t t = 0; t < nt; ++t)
+        jobs[t] = (proto_job){c, scales, xi, J, nb, stride, t, nt, rows};
+    for (size_t t = 1; t < nt; ++t)
+        started[t] = pthread_create(&th[t], NULL, proto_rows_run, &jobs[t]) == 0;
+    proto_rows_run(&jobs[0]);
+    for (size_t t = 1; t < nt; ++t) {
+        if (started[t]) pthread_join(th[t], NULL);
+        else proto_rows_run(&jobs[t]);       /* no thread: fill its rows here */
+    }
+}
+
 static void bank_free(fafi_cwt_bank *b) {
     if (!b) return;
     free(b->scales); free(b->freqs); free(b->l2); free(b->A); free(b->dual);
@@ -172,7 +266,8 @@ static fafi_cwt_bank *bank_build(const faf_cwt_config *cfg) {
         fafi_err("cwt: FAF_CWT_FLAG_FIXED_LOWPASS is reserved");
         return NULL;
     }
-    double xi0 = c.center_kind == FAF_CWT_CENTER_ENERGY ? energy_omega(&c) : peak_omega(&c);
+    double xi0, adm_c;
+    prototype_sums(&c, &xi0, &adm_c);
     if (!(xi0 > 0.0) || xi0 != xi0) {
         fafi_err("cwt: prototype center frequency is not finite");
         return NULL;
@@ -219,12 +314,7 @@ static fafi_cwt_bank *bank_build(const faf_cwt_config *cfg) {
         b->scales[j] = xi0 / (2.0 * M_PI * f / c.fs);
     }
     /* L2 shadow: rows coarse -> fine after the lowpass slot */
-    for (size_t j = 0; j < J; ++j) {
-        double *row = b->l2 + (off + j) * stride;
-        for (size_t k = 0; k < nb; ++k) row[k] = proto(b->scales[j] * xi[k], &c);
-        row[0] = 0.0;
-        if (nb > 1) row[nb - 1] = 0.0;
-    }
+    proto_rows(&c, b->scales, J, xi, nb, b->l2 + off * stride, stride);
     if (c.scale_kind == FAF_CWT_SCALE_GEOMETRIC) {
         /* rescale so the plateau of sum_j psi^2 (bins within 5% of its peak) is 1 */
         double amax = 0.0, mean = 0.0;
@@ -344,7 +434,7 @@ static fafi_cwt_bank *bank_build(const faf_cwt_config *cfg) {
         if (v > mdc) mdc = v;
     }
     rep->max_dc_wavelet = mdc;
-    rep->admissibility_C = admissibility(&c, xi0);
+    rep->admissibility_C = adm_c;
     rep->passed = (mx <= c.lp_alpha && rep->mean_abs_dev <= c.lp_beta && cnt > 0) ? 1 : 0;
     free(xi);
     const int strict = (c.flags & FAF_CWT_FLAG_VALIDATE_STRICT) &&

@@ -242,10 +242,19 @@ static void *tab_put(int kind, size_t n, int inv, const float *host, size_t floa
 /* exp(sign * 2 pi i * m / n) with the phase reduced by symmetry so the FP64
  * argument stays within [0, pi/4] */
 static void unit_root(size_t m, size_t n, int inv, double *c, double *s) {
-    /* angle = 2 pi m / n; use the octant of 8m relative to n */
-    m %= n;
-    long double a = 2.0L * (long double)M_PI * (long double)m / (long double)n;
-    double cc = (double)cosl(a), ss = (double)sinl(a);
+    /* angle = 2 pi m / n = (pi / 4n) A with A = 8 (m mod n) in [0, 8n):
+     * folded in exact integers into [0, pi/4] (sin and cos signs and a
+     * swap), so the FP64 sin/cos see a small argument with one rounding */
+    uint64_t A = 8 * (uint64_t)(m % n), N = n;
+    int neg_s = 0, neg_c = 0, swap = 0;
+    if (A > 4 * N) { A = 8 * N - A; neg_s = 1; }       /* (pi, 2pi): sin < 0 */
+    if (A > 2 * N) { A = 4 * N - A; neg_c = 1; }       /* (pi/2, pi]: cos < 0 */
+    if (A > N) { A = 2 * N - A; swap = 1; }            /* (pi/4, pi/2]: cos <-> sin */
+    const double x = (M_PI / 4.0) * ((double)A / (double)N);
+    double cc = cos(x), ss = sin(x);
+    if (swap) { double t = cc; cc = ss; ss = t; }
+    if (neg_c) cc = -cc;
+    if (neg_s) ss = -ss;
     *c = cc;
     *s = inv ? ss : -ss;
 }
