@@ -50,33 +50,36 @@ FAF_DEV void rg_mix(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &
     ch.run_lazy(src, lds, dst, lt, active, (const float2 *)pl->fwd.tw);
 }
 
-/* Rader for prime P with compile-time passes over the P - 1 inner points;
- * the same steps as rader_body (faf_engine.h) */
+/* Rader for prime P with compile-time passes over the P - 1 inner points:
+ * the steps of rader_body (faf_engine.h) in one LDS buffer per signal.  The
+ * plane of P - 1 points has lpad(P - 1) + 1 slots, room for the P samples:
+ * the signal is staged there, the first forward pass gathers from it and
+ * (after its barrier) writes the pass in place, and the last inverse pass
+ * scatters y[rout[j]] back into it, again after its barrier; y[0] goes
+ * straight to the output. */
 template <int P, int V>
 FAF_DEV void rg_rader(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst,
                       int sid, int lt, int active) {
     constexpr int T = RaderSeq<P, V>::T;
     using Seq = typename RaderSeq<P, V>::type;
     const fafd_rader &R = pl->rader;
-    const int per = pl->fwd.plane + lpad(P) + 1;
-    Lds xa{faf_smem + sid * per};
-    Lds aa{faf_smem + sid * per + lpad(P) + 1};
+    Lds aa{faf_smem + sid * pl->fwd.plane};
     if (active)
-        for (int i = lt; i < P; i += T) xa.st(i, src.ld(i));
+        for (int i = lt; i < P; i += T) aa.st(i, src.ld(i));
     __syncthreads();
-    const float2 x0 = active ? xa.ld(0) : make_float2(0.0f, 0.0f);
-    GatherLds g{xa, R.rin};
+    const float2 x0 = active ? aa.ld(0) : make_float2(0.0f, 0.0f);
+    GatherLds g{aa, R.rin};
     typename MixFFT<false, P - 1, T, Seq>::Chain fw;
     fw.run_lazy(g, aa, aa, lt, active, (const float2 *)pl->fwd.tw);
     __syncthreads();
-    if (active && lt == 0) xa.st(0, cadd(x0, aa.ld(0)));   /* DC = x0 + sum x[1..] */
+    if (active && lt == 0) dst.st(0, cadd(x0, aa.ld(0)));   /* DC = x0 + sum x[1..] */
     MulLds mb{aa, (const float2 *)R.b};
-    ScatterLds sc{xa, R.rout, x0};
+    ScatterLds sc{aa, R.rout, x0};
     typename MixFFT<true, P - 1, T, Seq>::Chain bw;
     bw.run_lazy(mb, aa, sc, lt, active, (const float2 *)pl->inv.tw);
     __syncthreads();
     if (active)
-        for (int i = lt; i < P; i += T) dst.st(i, xa.ld(i));
+        for (int i = 1 + lt; i < P; i += T) dst.st(i, aa.ld(i));
 }
 
 /* GEN: also the runtime Stockham, Rader and Bluestein paths.  Without them

@@ -198,13 +198,15 @@ static int plan_length(fafh_plan *p, size_t n, uint32_t flags, const rg_choice *
         }
         d->kind = kind;
         d->fwd = p->a; d->inv = p->b; d->rader = p->rader;
+        /* the runtime Rader path stages the signal in LDS beside the plane;
+         * the compile-time one stages, gathers and scatters in the plane */
+        *per = (size_t)p->a.plane + (kind == FAFD_KIND_RADER ? n + (n >> 4) + 1 : 0);
         if (kind == FAFD_KIND_RADER_CT) {
             d->variant = var;
             d->path = fafk_rg_path(FAFD_KIND_RADER_CT, (int)n, var);
             d->fwd.threads = rt;
             if (nr > 0) set_order(&d->fwd, ord, nr);
         }
-        *per = (size_t)p->a.plane + n + (n >> 4) + 1;
     } else if (p->kind == PK_BLUE) {
         if (want >= 0 && want != FAFD_KIND_BLUESTEIN) {
             fafi_err("ragged: length %zu runs through Bluestein only", n);
