@@ -7,7 +7,7 @@ inputs resident in HBM.  value = algorithmic bytes per step / step time,
 algorithmic bytes = batch * (4N + 8(N/2+1)).
 
 Also reported (the "extra" object): C2R and the other N of the same config,
-the fused notch pipeline, C2C N=4096 batch 256, the ragged persistent launch,
+the fused notch pipeline, C2C N=4096 batch 256, the ragged launch (one launch for 20,000 signals of uneven lengths),
 framed STFT, the CWT bank, the serving loop on the mixed request stream
 (p50/p99 latency, throughput against plain per-request launches), a
 device-to-device copy ceiling, and the plain per-call rate of the same
@@ -477,7 +477,7 @@ def run_extras(lib, fa, torch, np, timed, extra, only, err):
         lib.faf_destroy_transform(ti)
         lib.faf_destroy_transform(t)
 
-    # C3 ragged: 20,000 signals, one persistent launch
+    # C3 ragged: 20,000 signals, one launch
     if want("c3"):
         lens_set = [60, 360, 1000, 3000, 4800, 11, 17, 31, 97, 8192, 1009]
         rng = np.random.default_rng(0)
@@ -509,6 +509,8 @@ def run_extras(lib, fa, torch, np, timed, extra, only, err):
                 chosen[str(L_)] = {"kind": kinds.get(k_.value, k_.value), "spb": spb_.value,
                                    "order": list(order_[: no_.value])}
         extra["c3_tuned_choices"] = chosen
+        extra["c3_launch_shape"] = ("persistent workgroups" if lib.faf_ragged_persistent(r) == 1
+                                    else "one workgroup per unit")
         # every 50th signal and the first of each length, against FP64
         off = np.concatenate([[0], np.cumsum(lens)])
         pick = set(range(0, len(lens), 50))
