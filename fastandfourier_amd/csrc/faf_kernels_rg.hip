@@ -319,7 +319,9 @@ extern "C" int fafk_rg_ct_seq(int n, int v, int *order, int *nr) {
     if (v == 0) {
         switch (n) {
         case 256: case 512: case 1024: case 2048: case 4096: return n / 16;
-        case 8192: return rs8::T;
+        case 8192:                       /* the single pass: 8 x 32 x 32 in registers */
+            if (order && nr) { order[0] = 8; order[1] = 32; order[2] = 32; *nr = 3; }
+            return rs8::T;
         default: break;
         }
     }
