@@ -278,12 +278,16 @@ def main():
     z = torch.empty_like(x)
     ms_copy = timed(lambda: z.copy_(x), 20, 5)
     extra["d2d_copy_gbs"] = round(2 * x.numel() * 4 / (ms_copy * 1e-3) / 1e9, 1)
-    best = 0.0
-    for g in (1024, 2048, 4096, 8192):
-        ms_c = timed(lambda: fa.check(lib.faf_amd_copy_ceiling(z.data_ptr(), x.data_ptr(),
-                                                               x.numel() * 4, g), "copy"), 20, 5)
-        best = max(best, 2 * x.numel() * 4 / (ms_c * 1e-3) / 1e9)
+    best, best_nt = 0.0, 0
+    for nt in (0, 1):                       # plain and non-temporal stores
+        for g in (1024, 2048, 4096, 8192):
+            ms_c = timed(lambda: fa.check(lib.faf_amd_copy_ceiling_ex(
+                z.data_ptr(), x.data_ptr(), x.numel() * 4, g, nt), "copy"), 20, 5)
+            gb_c = 2 * x.numel() * 4 / (ms_c * 1e-3) / 1e9
+            if gb_c > best:
+                best, best_nt = gb_c, nt
     extra["copy_kernel_gbs"] = round(best, 1)
+    extra["copy_kernel_stores"] = "non-temporal" if best_nt else "plain"
     extra["headline_over_copy_kernel"] = round(gbs / best, 3)
     del z
     lib.faf_destroy_transform(t)
@@ -591,7 +595,8 @@ def run_extras(lib, fa, torch, np, timed, extra, only, err):
         t = lib.faf_create_cwt(C.byref(cc))
         rows = lib.faf_cwt_n_rows(t)
         segs = 1024
-        x = torch.randn((segs, n), device="cuda")
+        x = torch.randn((segs, n), device="cuda",
+                        generator=torch.Generator(device="cuda").manual_seed(4))
         z = torch.empty((segs, rows, n, 2), device="cuda")
         ms = timed(lambda: fa.check(lib.faf_execute_cwt_complex(t, z.data_ptr(), x.data_ptr(), segs, 0)), 20, 5)
         extra["cwt_1024seg_ms"] = round(ms, 3)

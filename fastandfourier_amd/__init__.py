@@ -176,6 +176,7 @@ _SIGS = {
     "faf_ragged_destroy": (None, [c_void_p]),
     "faf_ragged_autotune": (c_int, [c_void_p]),
     "faf_amd_copy_ceiling": (c_int, [c_void_p, c_void_p, c_size_t, c_int]),
+    "faf_amd_copy_ceiling_ex": (c_int, [c_void_p, c_void_p, c_size_t, c_int, c_int]),
     "faf_ragged_choice": (c_int, [c_void_p, c_size_t, POINTER(c_int), POINTER(c_int),
                                   POINTER(c_int), POINTER(c_int)]),
     "faf_ragged_set_choice": (c_int, [c_size_t, c_int, c_int, POINTER(c_int), c_int]),

@@ -190,15 +190,20 @@ int faf_amd_synchronize(void) {
 
 int faf_amd_device_ready(void) { return fafi_runtime_ready(); }
 
-int faf_amd_copy_ceiling(void *dst, const void *src, size_t bytes, int grid) {
+int faf_amd_copy_ceiling_ex(void *dst, const void *src, size_t bytes, int grid,
+                            int streaming) {
     if (!dst || !src) { fafi_err("copy: null buffer"); return -1; }
     if (fafi_runtime_ready()) return -1;
     if (grid <= 0) grid = (int)(8 * fafi_cus());
     void *stream = fafi_stream();
-    int rc = fafk_copy16(dst, src, bytes, grid, stream);
+    int rc = fafk_copy16(dst, src, bytes, grid, streaming, stream);
     if (!rc && fafi_sync_mode()) rc = fafk_sync(stream);
     if (rc) fafi_err("%s", fafk_last_error());
     return rc;
+}
+
+int faf_amd_copy_ceiling(void *dst, const void *src, size_t bytes, int grid) {
+    return faf_amd_copy_ceiling_ex(dst, src, bytes, grid, 0);
 }
 
 /* ------------------------------------------------------------------ */

@@ -79,7 +79,8 @@ void *fafk_capture_end(void *stream, int *nodes);
 int fafk_graph_launch(void *exec, void *stream);
 void fafk_graph_destroy(void *exec);
 int fafk_fill_u32(unsigned *p, int n, unsigned v, void *stream);
-int fafk_copy16(void *dst, const void *src, size_t bytes, int grid, void *stream);
+int fafk_copy16(void *dst, const void *src, size_t bytes, int grid, int streaming,
+                void *stream);
 int fafk_window_inter(const float *x, long in_dist, const float *w, long n, float *y,
                       long count, void *stream);
 

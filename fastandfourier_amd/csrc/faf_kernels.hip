@@ -288,8 +288,12 @@ extern "C" int fafk_fill_u32(unsigned *p, int n, unsigned v, void *stream) {
 /* Copy ceiling of the device: each thread moves four 16-byte words per
  * step (loads issued before stores), grid-stride over the buffer.  Used to
  * state what a pure read-once/write-once stream reaches on this board. */
+/* NT: non-temporal (streaming) stores, as the transforms use for rows
+ * written once */
+template <bool NT>
 __global__ __launch_bounds__(256) void k_copy16(const float4 *__restrict__ src,
                                                 float4 *__restrict__ dst, long n16) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
     const long stride = (long)gridDim.x * 256 * 4;
     for (long i = (long)blockIdx.x * 1024 + threadIdx.x; i < n16; i += stride) {
         float4 v[4];
@@ -298,17 +302,25 @@ __global__ __launch_bounds__(256) void k_copy16(const float4 *__restrict__ src,
             if (i + u * 256 < n16) v[u] = src[i + u * 256];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
-            if (i + u * 256 < n16) dst[i + u * 256] = v[u];
+            if (i + u * 256 < n16) {
+                if (NT) __builtin_nontemporal_store(*(const f4 *)&v[u], (f4 *)&dst[i + u * 256]);
+                else dst[i + u * 256] = v[u];
+            }
     }
 }
 
-extern "C" int fafk_copy16(void *dst, const void *src, size_t bytes, int grid, void *stream) {
+extern "C" int fafk_copy16(void *dst, const void *src, size_t bytes, int grid, int streaming,
+                           void *stream) {
     if ((bytes & 15) || (((uintptr_t)dst | (uintptr_t)src) & 15) || grid < 1) {
         snprintf(g_err, sizeof g_err, "copy: 16-byte aligned buffers and sizes only");
         return -1;
     }
-    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, S(stream), (const float4 *)src,
-                       (float4 *)dst, (long)(bytes / 16));
+    if (streaming)
+        hipLaunchKernelGGL(k_copy16<true>, dim3(grid), dim3(256), 0, S(stream),
+                           (const float4 *)src, (float4 *)dst, (long)(bytes / 16));
+    else
+        hipLaunchKernelGGL(k_copy16<false>, dim3(grid), dim3(256), 0, S(stream),
+                           (const float4 *)src, (float4 *)dst, (long)(bytes / 16));
     LAUNCH_CHECK("k_copy16");
     return 0;
 }

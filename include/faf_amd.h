@@ -28,6 +28,10 @@ int faf_amd_device_ready(void);              /* 0 if a gfx950 device is usable *
  * reaches on this board, the practical ceiling the transforms are measured
  * against beside the 8 TB/s peak.  Device buffers, 16-byte aligned. */
 int faf_amd_copy_ceiling(void *dst, const void *src, size_t bytes, int grid);
+/* the same copy; streaming != 0 stores with non-temporal hints, as the
+ * transforms do for rows they write once */
+int faf_amd_copy_ceiling_ex(void *dst, const void *src, size_t bytes, int grid,
+                            int streaming);
 
 /* ---- graph capture of executes (thread-local) ----
  * Between faf_graph_begin and faf_graph_end, faf_execute* and
