@@ -265,6 +265,18 @@ struct GInterOut {
  * irfft 1024 (5.33 -> 5.55 TB/s) and the CWT rows gain; the ragged launch,
  * whose rows end on partial lines, lost 10% with them and keeps plain
  * stores.  Same values either way. */
+/* One streaming (non-temporal) store of a float / float2.  The
+ * single-pass kernels whose output rows go out in whole lines use them
+ * (A/B, same bits: irfft 4096 5.43 -> 5.72 TB/s, irfft 16384 5.34 -> 5.49,
+ * irfft 65536 3.79 -> 4.02, C2C 32768 4.56 -> 4.70, notch +1%); the rfft
+ * single passes store mirrored bin pairs on partial lines and lost 8-14%
+ * with them, C2C 8192 1.5%: those keep plain stores. */
+FAF_DEV void nt_st(float *p, float v) { __builtin_nontemporal_store(v, p); }
+FAF_DEV void nt_st(float2 *p, float2 v) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+
 struct GSplitOutNT {
     float *re, *im;
     float s;

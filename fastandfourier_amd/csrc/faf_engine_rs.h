@@ -340,7 +340,7 @@ __global__ __launch_bounds__(1024) void kr_c2r(const float2 *tw, const float2 *p
         fft32768<true>(v, lds, tw, b1, tid);
         float2 *yo = (float2 *)(y + s * out_dist);
 #pragma unroll
-        for (int j = 0; j < 32; ++j) yo[tid + T * j] = cscale(v[j], scale);
+        for (int j = 0; j < 32; ++j) nt_st(&yo[tid + T * j], cscale(v[j], scale));
     }
 }
 
@@ -372,8 +372,8 @@ __global__ __launch_bounds__(1024) void kr_c2c(const float2 *tw, const float *in
 #pragma unroll
         for (int j = 0; j < 32; ++j) {
             const int k = tid + T * j;
-            if (INTER_OUT) oi[k] = cscale(v[j], scale);
-            else { ore[k] = v[j].x * scale; oim[k] = v[j].y * scale; }
+            if (INTER_OUT) nt_st(&oi[k], cscale(v[j], scale));
+            else { nt_st(&ore[k], v[j].x * scale); nt_st(&oim[k], v[j].y * scale); }
         }
     }
 }
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(256) void kr8_c2r(const float2 *tw, const float2 *p
     fft8192<true>(v, lds, tw, b1, tid);
     float2 *yo = (float2 *)(y + s * out_dist);
 #pragma unroll
-    for (int j = 0; j < 32; ++j) yo[tid + T * j] = cscale(v[j], scale);
+    for (int j = 0; j < 32; ++j) nt_st(&yo[tid + T * j], cscale(v[j], scale));
 }
 
 /* irfft N = 4096 in one pass, one wave per signal */
@@ -639,7 +639,7 @@ __global__ __launch_bounds__(64) void kr2_c2r(const float2 *tw, const float2 *pr
     fft2048<true>(v, lds, tw, b1, tid);
     float2 *yo = (float2 *)(y + s * out_dist);
 #pragma unroll
-    for (int j = 0; j < 32; ++j) yo[tid + T * j] = cscale(v[j], scale);
+    for (int j = 0; j < 32; ++j) nt_st(&yo[tid + T * j], cscale(v[j], scale));
 }
 
 #ifndef FAF_RS_CORE_ONLY   /* set by files that use only the cores above */
@@ -704,7 +704,7 @@ __global__ __launch_bounds__(64) void kr2_pipe(const float2 *twf, const float2 *
     fft2048<true>(v, lds, twi, b1, tid);
     float2 *yo = (float2 *)(y + s * out_dist);
 #pragma unroll
-    for (int j = 0; j < 32; ++j) yo[tid + T * j] = cscale(v[j], scale);
+    for (int j = 0; j < 32; ++j) nt_st(&yo[tid + T * j], cscale(v[j], scale));
 }
 
 #endif
