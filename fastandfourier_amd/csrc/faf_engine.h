@@ -311,6 +311,20 @@ struct GPairIn {
     FAF_DEV float2 ld(int i) const { return make_float2(x[2 * i], x[2 * i + 1]); }
 };
 
+/* GPairIn with streaming (non-temporal) loads: the headline rfft reads its
+ * 1 GiB of input once (5.33-5.35 -> 5.40-5.45 TB/s with the streaming row
+ * stores; same bits).  Elsewhere streaming loads measured 1-8% slower
+ * (irfft, C2C) and are not used. */
+struct GPairInNT {
+    const float *x;
+    static constexpr bool is_lds = false;
+    FAF_DEV float2 ld(int i) const {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        const f2 v = __builtin_nontemporal_load((const f2 *)(x + 2 * i));
+        return make_float2(v.x, v.y);
+    }
+};
+
 struct GPairOut {            /* C2R unpacking y[2i] = Re z, y[2i+1] = Im z */
     float *y;
     float s;

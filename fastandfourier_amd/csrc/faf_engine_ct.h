@@ -258,7 +258,7 @@ void kc_r2c(fafd_fft P, const float2 *post, const float *x, long in_dist, float 
     const int active = g < count;
     const long sig = active ? g : 0;
     Lds lds{faf_smem + sid * P.plane};
-    GPairIn src{x + sig * in_dist};
+    GPairInNT src{x + sig * in_dist};
     typename Ct<false, NC>::Chain ch;
     ch.load_tw(lt, active, (const float2 *)P.tw);
     /* the post-pass super-twiddles too are loaded up front, except at
