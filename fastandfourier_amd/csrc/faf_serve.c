@@ -19,7 +19,10 @@
  * Several slots, each with its own stream, let launches of successive
  * windows overlap on the device while new requests keep arriving.  Plans
  * (twiddles, Kiss tables, windows, CWT filter banks) are built once per
- * (kind, length) and stay resident in a device plan table.
+ * (kind, length) and stay resident in a device plan table.  A submission
+ * whose plan does not exist yet hands the build to a small pool of builder
+ * threads and returns at once; its request waits (parked) until the plan
+ * is ready, while requests with ready plans go ahead of it.
  *
  * The reference library has no serving layer; this file is new.
  */
@@ -40,6 +43,7 @@
 #define SV_CWT_ROWS_PER_UNIT 4
 #define SV_BLOCK_SMALL 512
 #define SV_ALIGN 256
+#define SV_BUILDERS 4            /* plan-building threads */
 
 enum { ST_QUEUED = 0, ST_ADMITTED, ST_LAUNCHED, ST_DONE, ST_FAILED };
 
@@ -83,6 +87,8 @@ typedef struct sentry {
     size_t ws_bytes;         /* long transforms: workspace of one request */
     uint64_t stamp;          /* worker: batch that last used it */
     int bucket;
+    _Atomic int state;       /* 0 being built, 1 ready, -1 failed (err says why) */
+    char err[160];
 } sentry;
 
 typedef struct srec {
@@ -97,6 +103,7 @@ typedef struct srec {
     double io_bytes;
     int waited;
     int freed;
+    int sized;               /* out/blk/io bytes known (its plan was ready) */
     _Atomic int state;
 } srec;
 
@@ -122,6 +129,14 @@ struct faf_server {
     int hash[2 * SV_MAX_ENTRIES];   /* open addressing: entry index + 1, 0 empty */
     fafd_splan *splans_dev;
     int nsplan;
+    pthread_mutex_t splan_mu;        /* the device plan table's fill level */
+    pthread_cond_t plan_cv;          /* an entry finished building (plan_mu) */
+    /* plan builds handed to the builder threads (plan_mu) */
+    pthread_cond_t build_cv;
+    sentry **bq;
+    int bq_head, bq_n, bq_cap;
+    pthread_t builders[SV_BUILDERS];
+    int nbuilders, build_stop;
     /* request records, in pages of SV_PAGE tickets; page number pg lives in
      * slot pg % SV_MAX_PAGES, and page_no[] says which page a slot holds
      * (-1 when empty).  A slot is recycled once every request of its page
@@ -136,6 +151,7 @@ struct faf_server {
     pthread_cond_t in_cv;
     ring inbox, rel;
     int sleeping;
+    uint64_t plans_built;    /* builds finished (in_mu): wakes the worker */
     /* completion */
     pthread_mutex_t done_mu;
     pthread_cond_t done_cv;
@@ -150,6 +166,8 @@ struct faf_server {
     frange *fr;
     int nfr, frcap;
     ring pend, ready;
+    ring parked;             /* requests whose plan is still being built */
+    uint64_t plans_seen;     /* plans_built when parked was last rescanned */
     slot slots[SV_MAX_SLOTS];
     int nslots;
     _Atomic int plain;       /* one launch + synchronize per request */
@@ -291,13 +309,18 @@ static size_t stft_frames(const faf_server *s, size_t n) {
 static double nlogn(double n) { return n * log2(n > 2 ? n : 2); }
 
 static int upload_splan(faf_server *s, const fafd_splan *sp) {
-    if (s->nsplan >= SV_MAX_ENTRIES) { fafi_err("serve: plan table full"); return -1; }
-    int idx = s->nsplan;
+    pthread_mutex_lock(&s->splan_mu);
+    if (s->nsplan >= SV_MAX_ENTRIES) {
+        pthread_mutex_unlock(&s->splan_mu);
+        fafi_err("serve: plan table full");
+        return -1;
+    }
+    int idx = s->nsplan++;
+    pthread_mutex_unlock(&s->splan_mu);
     if (fafk_h2d(s->splans_dev + idx, sp, sizeof *sp, NULL) || fafk_sync(NULL)) {
         fafi_err("serve: %s", fafk_last_error());
         return -1;
     }
-    s->nsplan++;
     return idx;
 }
 
@@ -562,43 +585,146 @@ static unsigned hslot(int kind, size_t key) {
     return (unsigned)(h >> 40) & (2 * SV_MAX_ENTRIES - 1);
 }
 
-/* cached entry of (kind, n); STFT entries are keyed by the window */
-static sentry *entry_get(faf_server *s, int kind, size_t n) {
-    size_t key = kind == FAF_REQ_STFT ? s->cfg.stft_win : n;
-    pthread_mutex_lock(&s->plan_mu);
+/* the entry of (kind, key) in the table, or NULL; *slot receives the empty
+ * hash slot the probe ended on.  plan_mu held. */
+static sentry *entry_find(faf_server *s, int kind, size_t key, unsigned *slot) {
     unsigned h = hslot(kind, key);
     for (; s->hash[h]; h = (h + 1) & (2 * SV_MAX_ENTRIES - 1)) {
         sentry *e = s->ent[s->hash[h] - 1];
-        if (e->kind == kind && e->n == key) {
-            pthread_mutex_unlock(&s->plan_mu);
-            return e;
-        }
+        if (e->kind == kind && e->n == key) return e;
     }
-    sentry *e = NULL;
-    if (s->nent >= SV_MAX_ENTRIES) {
-        fafi_err("serve: too many distinct (kind, length) plans");
-    } else if (kind < FAF_REQ_RFFT || kind > FAF_REQ_CWT) {
+    *slot = h;
+    return NULL;
+}
+
+/* the checks a request gets at once, before any plan is built */
+static int request_ok(faf_server *s, int kind, size_t n) {
+    if (kind < FAF_REQ_RFFT || kind > FAF_REQ_CWT) {
         fafi_err("serve: unknown request kind %d", kind);
-    } else if (n < 2 || n > ((size_t)1 << 16) * (kind == FAF_REQ_STFT ? 1024 : 1)) {
+        return 0;
+    }
+    if (n < 2 || n > ((size_t)1 << 16) * (kind == FAF_REQ_STFT ? 1024 : 1)) {
         fafi_err("serve: length %zu out of range", n);
-    } else {
+        return 0;
+    }
+    if (kind == FAF_REQ_RFFT && (n < 4 || (n & 1))) {
+        fafi_err("serve: rfft length %zu must be even and >= 4", n);
+        return 0;
+    }
+    return 1;
+}
+
+/* builds e outside plan_mu and publishes the outcome */
+static void build_publish(faf_server *s, sentry *e) {
+    const int rc = build_entry(s, e);
+    if (rc) snprintf(e->err, sizeof e->err, "%s", faf_get_error());
+    pthread_mutex_lock(&s->plan_mu);
+    atomic_store(&e->state, rc ? -1 : 1);
+    pthread_cond_broadcast(&s->plan_cv);
+    pthread_mutex_unlock(&s->plan_mu);
+    pthread_mutex_lock(&s->in_mu);           /* the worker rescans its parked requests */
+    s->plans_built++;
+    if (s->sleeping) pthread_cond_signal(&s->in_cv);
+    pthread_mutex_unlock(&s->in_mu);
+}
+
+static void *builder_main(void *arg) {
+    faf_server *s = arg;
+    pthread_mutex_lock(&s->plan_mu);
+    for (;;) {
+        while (!s->bq_n && !s->build_stop) pthread_cond_wait(&s->build_cv, &s->plan_mu);
+        if (!s->bq_n) break;                 /* stopping and nothing left */
+        sentry *e = s->bq[s->bq_head];
+        s->bq_head = (s->bq_head + 1) % s->bq_cap;
+        s->bq_n--;
+        pthread_mutex_unlock(&s->plan_mu);
+        build_publish(s, e);
+        pthread_mutex_lock(&s->plan_mu);
+    }
+    pthread_mutex_unlock(&s->plan_mu);
+    return NULL;
+}
+
+/* Entry of (kind, n), created when missing (state 0, being built); STFT
+ * entries are keyed by the window.  With wait, a missing entry is built on
+ * this thread and an entry being built elsewhere is waited for, so a ready
+ * entry (or NULL with the build's error) comes back; without wait, a
+ * missing entry goes to the builder threads and may come back unbuilt. */
+static sentry *entry_acquire(faf_server *s, int kind, size_t n, int wait) {
+    if (!request_ok(s, kind, n)) return NULL;
+    size_t key = kind == FAF_REQ_STFT ? s->cfg.stft_win : n;
+    unsigned h = 0;
+    pthread_mutex_lock(&s->plan_mu);
+    sentry *e = entry_find(s, kind, key, &h);
+    int mine = 0;
+    if (!e) {
+        if (s->nent >= SV_MAX_ENTRIES) {
+            pthread_mutex_unlock(&s->plan_mu);
+            fafi_err("serve: too many distinct (kind, length) plans");
+            return NULL;
+        }
         e = calloc(1, sizeof *e);
-        if (e) {
-            e->kind = kind;
-            e->n = key;
-            if (build_entry(s, e)) {
-                fafi_plan_free(e->plan);
-                free(e);
-                e = NULL;
-            } else {
-                s->ent[s->nent++] = e;
-                s->hash[h] = s->nent;     /* h is the empty slot the probe ended on */
+        if (!e) {
+            pthread_mutex_unlock(&s->plan_mu);
+            fafi_err("serve: alloc failed");
+            return NULL;
+        }
+        e->kind = kind;
+        e->n = key;
+        s->ent[s->nent++] = e;
+        s->hash[h] = s->nent;
+        mine = 1;
+        if (!wait) {
+            /* to the builder threads (started on the first such build) */
+            int queued = 0;
+            if (s->bq_n == s->bq_cap) {
+                int nc = s->bq_cap ? 2 * s->bq_cap : 64;
+                sentry **q = malloc((size_t)nc * sizeof *q);
+                if (q) {
+                    for (int i = 0; i < s->bq_n; ++i) q[i] = s->bq[(s->bq_head + i) % s->bq_cap];
+                    free(s->bq);
+                    s->bq = q;
+                    s->bq_head = 0;
+                    s->bq_cap = nc;
+                }
+            }
+            if (s->bq_n < s->bq_cap) {
+                s->bq[(s->bq_head + s->bq_n) % s->bq_cap] = e;
+                s->bq_n++;
+                while (s->nbuilders < SV_BUILDERS &&
+                       s->nbuilders < s->bq_n &&
+                       pthread_create(&s->builders[s->nbuilders], NULL, builder_main, s) == 0)
+                    s->nbuilders++;
+                if (s->nbuilders > 0) {
+                    pthread_cond_signal(&s->build_cv);
+                    queued = 1;
+                } else {
+                    s->bq_n--;               /* no builder thread: build here */
+                }
+            }
+            if (queued) {
+                pthread_mutex_unlock(&s->plan_mu);
+                return e;
             }
         }
     }
+    if (mine) {
+        pthread_mutex_unlock(&s->plan_mu);
+        build_publish(s, e);
+        pthread_mutex_lock(&s->plan_mu);
+    }
+    if (wait)
+        while (atomic_load(&e->state) == 0) pthread_cond_wait(&s->plan_cv, &s->plan_mu);
+    const int st = atomic_load(&e->state);
     pthread_mutex_unlock(&s->plan_mu);
+    if (st < 0) {
+        fafi_err("%s", e->err[0] ? e->err : "serve: plan build failed");
+        return NULL;
+    }
     return e;
 }
+
+static sentry *entry_get(faf_server *s, int kind, size_t n) { return entry_acquire(s, kind, n, 1); }
 
 static size_t result_bytes(faf_server *s, const sentry *e, size_t n) {
     switch (e->kind) {
@@ -659,11 +785,60 @@ static int take_inbox(faf_server *s) {
     return did;
 }
 
+/* result, arena block and I/O bytes of r, from its (ready) plan */
+static void size_request(faf_server *s, srec *r) {
+    r->out_bytes = result_bytes(s, r->e, r->n);
+    r->blk_bytes = al(r->out_bytes) + al(r->e->ws_bytes);
+    r->io_bytes = input_bytes(r->kind, r->n) + (double)r->out_bytes;
+    r->sized = 1;
+}
+
+/* Parked requests whose plan finished building (or failed) go back to the
+ * admission queue; run when a build has finished since the last scan. */
+static int unpark(faf_server *s) {
+    pthread_mutex_lock(&s->in_mu);
+    const uint64_t built = s->plans_built;
+    pthread_mutex_unlock(&s->in_mu);
+    if (!s->parked.n || built == s->plans_seen) return 0;
+    s->plans_seen = built;
+    int did = 0;
+    for (size_t i = 0, m = s->parked.n; i < m; ++i) {
+        int64_t t = ring_pop(&s->parked);
+        if (atomic_load(&rec(s, t)->e->state) != 0) {
+            if (!ring_push(&s->pend, t)) { did = 1; continue; }
+        }
+        ring_push(&s->parked, t);            /* still building (or no room to move) */
+    }
+    return did;
+}
+
 static int admit(faf_server *s) {
     int did = 0;
     while (s->pend.n) {
         int64_t t = ring_at(&s->pend, 0);
         srec *r = rec(s, t);
+        if (!r->sized) {
+            const int st = atomic_load(&r->e->state);
+            if (st == 0) {                   /* its plan is being built: step aside */
+                ring_pop(&s->pend);
+                if (ring_push(&s->parked, t)) {
+                    set_err(s, "serve: alloc failed");
+                    r->freed = 1;
+                    finish(s, &t, 1, ST_FAILED);
+                }
+                did = 1;
+                continue;
+            }
+            if (st < 0) {
+                ring_pop(&s->pend);
+                set_err(s, r->e->err[0] ? r->e->err : "serve: plan build failed");
+                r->freed = 1;
+                finish(s, &t, 1, ST_FAILED);
+                did = 1;
+                continue;
+            }
+            size_request(s, r);
+        }
         if (r->blk_bytes > s->arena_bytes) {
             ring_pop(&s->pend);
             set_err(s, "serve: request larger than the HBM budget");
@@ -1009,14 +1184,20 @@ static void *worker_main(void *arg) {
     faf_server *s = arg;
     for (;;) {
         int did = take_inbox(s);
+        did |= unpark(s);
         did |= admit(s);
         did |= poll_slots(s);
         did |= maybe_launch(s);
         if (did) continue;
         if (atomic_load(&s->stop) && !s->ready.n && !has_busy(s)) {
-            /* shutting down: what is still waiting for budget fails */
+            /* shutting down: what is still waiting for budget or a plan fails */
             while (s->pend.n) {
                 int64_t t = ring_pop(&s->pend);
+                rec(s, t)->freed = 1;
+                finish(s, &t, 1, ST_FAILED);
+            }
+            while (s->parked.n) {
+                int64_t t = ring_pop(&s->parked);
                 rec(s, t)->freed = 1;
                 finish(s, &t, 1, ST_FAILED);
             }
@@ -1025,7 +1206,8 @@ static void *worker_main(void *arg) {
         if (!s->pend.n && !s->ready.n && !has_busy(s)) {
             /* idle: sleep until a submission, release or stop */
             pthread_mutex_lock(&s->in_mu);
-            if (!s->inbox.n && !s->rel.n && !atomic_load(&s->stop)) {
+            if (!s->inbox.n && !s->rel.n && !atomic_load(&s->stop) &&
+                (!s->parked.n || s->plans_built == s->plans_seen)) {
                 s->sleeping = 1;
                 pthread_cond_wait(&s->in_cv, &s->in_mu);
                 s->sleeping = 0;
@@ -1067,6 +1249,12 @@ void faf_server_destroy(faf_server *s) {
         pthread_mutex_unlock(&s->in_mu);
         pthread_join(s->th, NULL);
     }
+    pthread_mutex_lock(&s->plan_mu);          /* builders finish what they hold */
+    s->build_stop = 1;
+    pthread_cond_broadcast(&s->build_cv);
+    pthread_mutex_unlock(&s->plan_mu);
+    for (int i = 0; i < s->nbuilders; ++i) pthread_join(s->builders[i], NULL);
+    free(s->bq);
     for (int i = 0; i < s->nslots; ++i) {
         slot *sl = &s->slots[i];
         if (sl->stream) fafk_sync(sl->stream);
@@ -1084,11 +1272,14 @@ void faf_server_destroy(faf_server *s) {
     fafk_free(s->splans_dev);
     fafk_free(s->arena);
     free(s->fr);
-    free(s->inbox.v); free(s->rel.v); free(s->pend.v); free(s->ready.v);
+    free(s->inbox.v); free(s->rel.v); free(s->pend.v); free(s->ready.v); free(s->parked.v);
     free(s->bk_e); free(s->bk_cnt); free(s->bk_off);
     free(s->sigs); free(s->units); free(s->ucost); free(s->uorder); free(s->ulds);
     free(s->uthr); free(s->uphase);
     pthread_mutex_destroy(&s->plan_mu);
+    pthread_mutex_destroy(&s->splan_mu);
+    pthread_cond_destroy(&s->plan_cv);
+    pthread_cond_destroy(&s->build_cv);
     pthread_mutex_destroy(&s->in_mu);
     pthread_mutex_destroy(&s->done_mu);
     pthread_cond_destroy(&s->in_cv);
@@ -1114,6 +1305,9 @@ faf_server *faf_server_create(const faf_server_config *cfg) {
     if (!s) { fafi_err("serve: alloc failed"); return NULL; }
     s->cfg = c;
     pthread_mutex_init(&s->plan_mu, NULL);
+    pthread_mutex_init(&s->splan_mu, NULL);
+    pthread_cond_init(&s->plan_cv, NULL);
+    pthread_cond_init(&s->build_cv, NULL);
     pthread_mutex_init(&s->in_mu, NULL);
     pthread_mutex_init(&s->done_mu, NULL);
     pthread_cond_init(&s->in_cv, NULL);
@@ -1207,7 +1401,7 @@ static int64_t submit_at(faf_server *s, int kind, size_t n, const float *in, uin
                          uint64_t t_arr) {
     if (!s || !in) { fafi_err("serve: null argument"); return -1; }
     if (kind == FAF_REQ_STFT && n < 1) { fafi_err("serve: empty STFT record"); return -1; }
-    sentry *e = entry_get(s, kind, n);
+    sentry *e = entry_acquire(s, kind, n, 0);    /* may still be building */
     if (!e) return -1;
     /* the ticket is issued only once its record exists: a submission that
      * fails here leaves no ticket behind for drain or stats to wait on */
@@ -1238,9 +1432,7 @@ static int64_t submit_at(faf_server *s, int kind, size_t n, const float *in, uin
     r->n = n;
     r->in = in;
     r->e = e;
-    r->out_bytes = result_bytes(s, e, n);
-    r->blk_bytes = al(r->out_bytes) + al(e->ws_bytes);
-    r->io_bytes = input_bytes(kind, n) + (double)r->out_bytes;
+    if (atomic_load(&e->state) == 1) size_request(s, r);   /* else once its plan is built */
     r->t_arr = t_arr;
     atomic_store(&r->state, ST_QUEUED);
     if (ring_push(&s->inbox, t)) {

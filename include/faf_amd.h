@@ -247,7 +247,13 @@ int faf_server_set_window(faf_server *s, double window_us);
 int faf_server_prepare(faf_server *s, faf_req_kind kind, size_t n);
 /* bytes of the result of one (kind, n) request, 0 if not servable */
 size_t faf_server_result_bytes(faf_server *s, faf_req_kind kind, size_t n);
-/* returns a ticket >= 0, or -1 (faf_get_error) */
+/* Returns a ticket >= 0, or -1 (faf_get_error) for a request that is wrong
+ * on its face (unknown kind, length out of range, odd rfft, NULL input).
+ * A (kind, n) seen for the first time has its plan built on one of the
+ * server's builder threads: the call returns at once and the request waits
+ * for the plan while requests with ready plans go ahead of it.  Should that
+ * build fail, the ticket fails (faf_server_wait returns -1) and later
+ * submissions of that (kind, n) return -1 with the build's error. */
 int64_t faf_server_submit(faf_server *s, faf_req_kind kind, size_t n, const float *in,
                           uint32_t flags);
 int faf_server_poll(faf_server *s, int64_t ticket);   /* 1 done, 0 pending, -1 failed */

@@ -308,6 +308,58 @@ def test_serve_failed_submit_issues_no_ticket():
         lib.faf_server_destroy(s)
 
 
+def test_serve_plans_build_beside_ready_requests():
+    """Requests whose plans are new are submitted without waiting for the
+    build; they complete once their plan is ready, with the same results as
+    prepared ones, while a request with a ready plan goes ahead.  A plan whose
+    build fails (C2C 65521, a prime beyond the serving kernel's Rader and
+    Bluestein shapes) fails its ticket, and later submissions of that length
+    return -1 at once."""
+    say("case: serve plans built on builder threads")
+    lib = G.L()
+    rng = np.random.default_rng(33)
+    s = server(hbm_budget=1 << 30)
+    try:
+        fa.check(lib.faf_server_prepare(s, 0, 4096), "prepare")
+        cases = [("cwt", 8192), ("stft", 4000), ("c2c", 1000), ("rfft", 4096), ("c2c", 1009),
+                 ("rfft", 32768), ("cwt", 512)]
+        kid = {"rfft": 0, "c2c": 1, "stft": 2, "cwt": 3}
+        subs = []
+        for kind, n in cases:
+            x, dx = make_input(rng, kind, n)
+            t = lib.faf_server_submit(s, kid[kind], n, dx.data_ptr(), 0)
+            assert t >= 0, lib.faf_get_error()
+            subs.append((kind, n, x, dx, t))
+        xb, dxb = make_input(rng, "c2c", 65521)
+        tbad = lib.faf_server_submit(s, 1, 65521, dxb.data_ptr(), 0)
+        assert tbad >= 0, "the build of a new plan is not waited for at submission"
+        first = []
+        for kind, n, x, _, t in subs:
+            y = fetch(s, t)
+            err = rel_l2(y, truth(kind, n, x))
+            say(f"  {kind} {n}: rel l2 {err:.2e}")
+            assert err < bar(kind, n), (kind, n, err)
+            first.append(y)
+            fa.check(lib.faf_server_release(s, t), "release")
+        assert lib.faf_server_wait(s, tbad) == -1
+        assert lib.faf_server_submit(s, 1, 65521, dxb.data_ptr(), 0) == -1
+        assert lib.faf_get_error()
+        assert lib.faf_server_prepare(s, 1, 65521) == -1
+        fa.check(lib.faf_server_drain(s), "drain")
+        st = fa.FafServerStats()
+        lib.faf_server_stats_get(s, C.byref(st))
+        assert st.submitted == len(cases) + 1
+        assert st.completed == len(cases) and st.failed == 1
+        # the same requests with every plan now ready give the same bits
+        for (kind, n, _, dx, _), y0 in zip(subs, first):
+            t = lib.faf_server_submit(s, kid[kind], n, dx.data_ptr(), 0)
+            assert t >= 0, lib.faf_get_error()
+            assert np.array_equal(fetch(s, t), y0), (kind, n)
+            fa.check(lib.faf_server_release(s, t), "release")
+    finally:
+        lib.faf_server_destroy(s)
+
+
 @pytest.mark.parametrize("n", [16384, 32768, 65536])
 def test_cwt_long_segments_match_fp64(n):
     """CWT segments longer than one workgroup: rfft, spread, batched inverse."""
