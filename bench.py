@@ -514,7 +514,8 @@ def run_extras(lib, fa, torch, np, timed, extra, only, err):
                                    "order": list(order_[: no_.value])}
         extra["c3_tuned_choices"] = chosen
         extra["c3_launch_shape"] = ("persistent workgroups" if lib.faf_ragged_persistent(r) == 1
-                                    else "one workgroup per unit")
+                                    else "one workgroup per unit") + \
+            f", {lib.faf_ragged_block(r)} threads"
         # every 50th signal and the first of each length, against FP64
         off = np.concatenate([[0], np.cumsum(lens)])
         pick = set(range(0, len(lens), 50))

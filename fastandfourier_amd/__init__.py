@@ -182,6 +182,8 @@ _SIGS = {
     "faf_ragged_set_choice": (c_int, [c_size_t, c_int, c_int, POINTER(c_int), c_int]),
     "faf_ragged_persistent": (c_int, [c_void_p]),
     "faf_ragged_set_persistent": (c_int, [c_void_p, c_int]),
+    "faf_ragged_block": (c_int, [c_void_p]),
+    "faf_ragged_set_block": (c_int, [c_void_p, c_int]),
     "faf_amd_plan_shape": (c_int, [PT, POINTER(c_int), POINTER(c_int), POINTER(c_int),
                                    POINTER(c_int)]),
     "faf_stft_frames": (c_size_t, [PT, c_size_t]),

@@ -24,7 +24,12 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* workgroup sizes of the launch: 256 threads when every length of the
+ * batch runs on that many (one 8192-point signal per workgroup instead of
+ * two tied to the barriers of one; each length class measured as fast or
+ * faster, the C3 mix 170 -> 154 us), else 512 */
 #define RAGGED_BLOCK 512
+#define RAGGED_BLOCK_SMALL 256
 #define RAGGED_LDS_CAP (96 * 1024)
 /* with compile-time paths only, two workgroups share a CU's 160 KB */
 #define RAGGED_LDS_CAP_CT (80 * 1024 - 256)
@@ -48,6 +53,8 @@ struct faf_ragged {
     int nunits;
     int generic;              /* some length needs the runtime paths */
     int persist;              /* 1: persistent workgroups and a queue; 0: one per unit */
+    int block_want;           /* 0: 256 threads when every length fits, else 512 */
+    int block;                /* threads per workgroup of the launch */
     int grid;
     size_t lds;
 };
@@ -243,11 +250,11 @@ static const rg_choice *choice_for(size_t n, const rg_overrides *ov, rg_choice *
     return NULL;
 }
 
-/* Workgroups of the ragged kernel a CU holds at once: 8 waves each; the
- * lean kernel's 128 VGPRs allow 4 waves per SIMD (16 per CU), the one with
- * the runtime paths 2; and the CU's 160 KB of LDS, in 512-byte blocks. */
-static int ragged_per_cu(int generic, size_t lds) {
-    const int by_waves = generic ? 1 : 2;
+/* Workgroups of the ragged kernel a CU holds at once: the lean kernel's
+ * 128 VGPRs allow 4 waves per SIMD (16 per CU), the one with the runtime
+ * paths 2 (8 per CU); and the CU's 160 KB of LDS, in 512-byte blocks. */
+static int ragged_per_cu(int generic, size_t lds, int block) {
+    const int by_waves = (generic ? 8 : 16) / (block / 64);
     const size_t blk = (lds + 64 + 511) / 512 * 512;
     const int by_lds = (int)(FAFD_MAX_LDS_BYTES / blk);
     const int n = by_waves < by_lds ? by_waves : by_lds;
@@ -288,13 +295,24 @@ static int ragged_build(faf_ragged *r, const rg_overrides *ov) {
         want_spb[k] = c ? c->spb : 0;
         if (tab[k].kind != FAFD_KIND_CT && tab[k].kind != FAFD_KIND_RADER_CT) r->generic = 1;
     }
+    /* the workgroup size of the launch */
+    int maxt = 0;
+    for (int k = 0; k < nu && ok; ++k)
+        if (tab[k].threads > maxt) maxt = tab[k].threads;
+    const int block = r->block_want ? r->block_want
+                                    : (maxt <= RAGGED_BLOCK_SMALL ? RAGGED_BLOCK_SMALL : RAGGED_BLOCK);
+    if (ok && maxt > block) {
+        fafi_err("ragged: a length needs %d threads, more than a %d-thread workgroup", maxt, block);
+        ok = 0;
+    }
+    r->block = block;
     /* signals per unit: as many as the threads and the LDS cap allow, or
      * fewer when the choice asks */
     const size_t cap = r->generic ? RAGGED_LDS_CAP : RAGGED_LDS_CAP_CT;
     for (int k = 0; k < nu && ok; ++k) {
         fafd_rplan *d = &tab[k];
         const size_t per = pers[k];
-        int spb = RAGGED_BLOCK / d->threads;
+        int spb = block / d->threads;
         while (spb > 1 && (size_t)spb * per * 8 > cap) --spb;
         if (want_spb[k] > 0 && want_spb[k] < spb) spb = want_spb[k];
         if ((size_t)spb * per * 8 > FAFD_MAX_LDS_BYTES) {
@@ -306,7 +324,7 @@ static int ragged_build(faf_ragged *r, const rg_overrides *ov) {
         /* the 8192-point single pass exchanges through LDS in every signal
          * slot of the workgroup, active or not, so its units reserve all */
         const size_t slots = fafk_rg_ct_slots(d->len) > 0 && d->kind == FAFD_KIND_CT
-                                 ? (size_t)(RAGGED_BLOCK / d->threads) : (size_t)spb;
+                                 ? (size_t)(block / d->threads) : (size_t)spb;
         if (slots * per * 8 > lds_max) lds_max = slots * per * 8;
     }
     free(pers);
@@ -362,7 +380,7 @@ static int ragged_build(faf_ragged *r, const rg_overrides *ov) {
     r->lds = lds_max;
     /* persistent grid: the workgroups resident at once (each starts on its
      * own unit without a queue round trip), never more than the units */
-    int per_cu = ragged_per_cu(r->generic, r->lds);
+    int per_cu = ragged_per_cu(r->generic, r->lds, r->block);
     long g = (long)per_cu * (long)fafi_cus();
     r->grid = (int)(nuu < g ? nuu : g);
     if (r->grid < 1) r->grid = 1;
@@ -392,11 +410,12 @@ static int ragged_build(faf_ragged *r, const rg_overrides *ov) {
 }
 
 static faf_ragged *ragged_new(const size_t *lengths, size_t count, uint32_t flags,
-                              const rg_overrides *ov) {
+                              const rg_overrides *ov, int block_want) {
     faf_ragged *r = calloc(1, sizeof *r);
     if (!r) { fafi_err("ragged: alloc failed"); return NULL; }
     r->count = count;
     r->flags = flags;
+    r->block_want = block_want;
     r->lengths = malloc(count * sizeof(size_t));
     if (!r->lengths) { free(r); fafi_err("ragged: alloc failed"); return NULL; }
     memcpy(r->lengths, lengths, count * sizeof(size_t));
@@ -412,7 +431,7 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
     for (size_t i = 0; i < count; ++i)
         if (lengths[i] < 2) { fafi_err("ragged: length %zu is not a transform", lengths[i]); return NULL; }
     if (fafi_create_ready()) return NULL;
-    faf_ragged *r = ragged_new(lengths, count, flags, NULL);
+    faf_ragged *r = ragged_new(lengths, count, flags, NULL, 0);
     /* the reference's MEASURE flag (src/faf_core.c:539-565, called at :694)
      * asks for timing at create: tune the batch's lengths that have no
      * recorded choice yet */
@@ -426,7 +445,7 @@ faf_ragged *faf_ragged_create(const size_t *lengths, size_t count, uint32_t flag
 static int launch(faf_ragged *r, float *out_re, float *out_im, const float *in_re,
                   const float *in_im, void *stream) {
     int rc = fafk_ragged(r->d_plans, r->d_units, r->nunits, r->d_uoff, in_re, in_im,
-                         out_re, out_im, r->d_counter, r->grid, RAGGED_BLOCK, r->lds,
+                         out_re, out_im, r->d_counter, r->grid, r->block, r->lds,
                          r->generic, r->persist, stream);
     if (rc) fafi_err("ragged: %s", fafk_rg_last_error());
     return rc;
@@ -544,7 +563,8 @@ static int candidates(size_t n, uint32_t flags, rg_choice *out) {
     return nc;
 }
 
-/* the most signals of length n a unit can take under choice c */
+/* the most signals of length n a unit can take under choice c, in a
+ * batch of that length alone (which gets the workgroup size it fits) */
 static int max_spb(size_t n, uint32_t flags, const rg_choice *c, int generic) {
     fafh_plan *p = fafi_plan_new();
     fafd_rplan d;
@@ -553,7 +573,7 @@ static int max_spb(size_t n, uint32_t flags, const rg_choice *c, int generic) {
     fafi_plan_free(p);
     if (rc) return 0;
     const size_t cap = generic ? RAGGED_LDS_CAP : RAGGED_LDS_CAP_CT;
-    int spb = RAGGED_BLOCK / d.threads;
+    int spb = (d.threads <= RAGGED_BLOCK_SMALL ? RAGGED_BLOCK_SMALL : RAGGED_BLOCK) / d.threads;
     while (spb > 1 && (size_t)spb * per * 8 > cap) --spb;
     return spb;
 }
@@ -619,7 +639,7 @@ int faf_ragged_autotune(faf_ragged *r) {
                 c.spb = s;
             }
             rg_overrides ov = {&uniq[k], &c, 1};
-            faf_ragged *trial = ragged_new(one, m, r->flags, &ov);
+            faf_ragged *trial = ragged_new(one, m, r->flags, &ov, 0);
             if (!trial) continue;            /* this length cannot take that choice */
             float ms = time_ragged(trial, buf, stream);
             const int lean = !trial->generic;
@@ -652,16 +672,25 @@ int faf_ragged_autotune(faf_ragged *r) {
         const rg_overrides *ovs[3] = {&ov_best, &ov_lean, &ov_dflt};
         const rg_choice *win = NULL;
         float twin = 1e30f;
-        int win_persist = 0;
-        for (int c = 0; c < 3; ++c) {
-            faf_ragged *a = ragged_new(r->lengths, r->count, r->flags, ovs[c]);
-            for (int ps = 0; a && ps < 2; ++ps) {      /* one workgroup per unit, persistent */
-                a->persist = ps;
-                const float ta = time_ragged(a, buf, stream);
-                if (ta > 0.0f && ta < twin) { twin = ta; win = cand[c]; win_persist = ps; }
+        int win_persist = 0, win_block = 0;
+        for (int c = 0; c < 3; ++c)
+            for (int bw = 0, auto_block = 0; bw < 2; ++bw) {   /* the size that fits, then 512 */
+                if (bw && auto_block == RAGGED_BLOCK) break;     /* that was 512 already */
+                faf_ragged *a = ragged_new(r->lengths, r->count, r->flags, ovs[c],
+                                           bw ? RAGGED_BLOCK : 0);
+                if (a && !bw) auto_block = a->block;
+                for (int ps = 0; a && ps < 2; ++ps) {  /* one workgroup per unit, persistent */
+                    a->persist = ps;
+                    const float ta = time_ragged(a, buf, stream);
+                    if (ta > 0.0f && ta < twin) {
+                        twin = ta;
+                        win = cand[c];
+                        win_persist = ps;
+                        win_block = bw ? RAGGED_BLOCK : 0;
+                    }
+                }
+                faf_ragged_destroy(a);
             }
-            faf_ragged_destroy(a);
-        }
         if (!win) {
             fafi_err("ragged autotune: the batch did not run");
             rc = -1;
@@ -670,6 +699,7 @@ int faf_ragged_autotune(faf_ragged *r) {
             /* rebuild this plan with the winners */
             ragged_free_device(r);
             rg_overrides ov = {uniq, win, nu};
+            r->block_want = win_block;
             if (ragged_build(r, &ov)) rc = -1;
             r->persist = win_persist;
         }
@@ -768,6 +798,28 @@ int faf_ragged_set_choice(size_t len, int kind, int spb, const int *order, int n
 }
 
 int faf_ragged_persistent(const faf_ragged *r) { return r ? r->persist : -1; }
+
+int faf_ragged_block(const faf_ragged *r) { return r ? r->block : -1; }
+
+int faf_ragged_set_block(faf_ragged *r, int threads) {
+    if (!r) { fafi_err("ragged: null plan"); return -1; }
+    if (threads != 0 && threads != RAGGED_BLOCK_SMALL && threads != RAGGED_BLOCK) {
+        fafi_err("ragged: workgroup size %d is not 0, 256 or 512", threads);
+        return -1;
+    }
+    if (fafi_create_ready()) return -1;
+    const int old = r->block_want;
+    ragged_free_device(r);
+    r->block_want = threads;
+    if (ragged_build(r, NULL) == 0) return 0;
+    char msg[256];
+    snprintf(msg, sizeof msg, "%s", faf_get_error());
+    ragged_free_device(r);
+    r->block_want = old;
+    if (ragged_build(r, NULL)) return -1;
+    fafi_err("%s", msg);
+    return -1;
+}
 
 int faf_ragged_set_persistent(faf_ragged *r, int on) {
     if (!r) { fafi_err("ragged: null plan"); return -1; }

@@ -125,6 +125,14 @@ int faf_ragged_set_choice(size_t len, int kind, int spb, const int *order, int n
  * faf_ragged_autotune times both on the batch and keeps the faster. */
 int faf_ragged_persistent(const faf_ragged *r);
 int faf_ragged_set_persistent(faf_ragged *r, int on);
+/* Threads per workgroup of r's launch: 256 when every length of the batch
+ * runs on that many (one 8192-point signal per workgroup), else 512.
+ * faf_ragged_set_block takes 0 (that automatic size), 256 or 512 and
+ * rebuilds r's units; when a length needs more threads it fails and r stays
+ * as it was.  faf_ragged_autotune times both sizes on the batch and keeps
+ * the faster. */
+int faf_ragged_block(const faf_ragged *r);
+int faf_ragged_set_block(faf_ragged *r, int threads);
 
 /* ---- framed real STFT ----
  * faf_create_stft with cfg.layout == FAF_LAYOUT_REAL (an error in the

@@ -574,9 +574,11 @@ def test_c3_ragged_persistent_launch():
     (C3_LENGTHS, 0),
     ([4050, 128, 13, 19, 2000, 23, 97, 8192, 24, 1009, 60, 4096], 1)])
 def test_ragged_launch_shapes_agree(lengths, flags):
-    """One workgroup per unit and persistent workgroups run the same unit code:
-    every signal has the same bits in both shapes (lean kernel, and the one
-    with the runtime paths), over several launches of each."""
+    """One workgroup per unit and persistent workgroups, of 256 or 512 threads,
+    run the same unit code: every signal has the same bits in every shape
+    (lean kernel, and the one with the runtime paths), over several launches
+    of each.  A batch with a length that needs more than 256 threads refuses
+    the smaller workgroup and keeps running as before."""
     say(f"case: ragged launch shapes {lengths[:3]}...")
     lib = G.L()
     rng = np.random.default_rng(5)
@@ -588,8 +590,21 @@ def test_ragged_launch_shapes_agree(lengths, flags):
         tot = lib.faf_ragged_total(r)
         x = (rng.uniform(-1, 1, tot) + 1j * rng.uniform(-1, 1, tot)).astype(np.complex64)
         dxr, dxi = G.dev(x.real.copy()), G.dev(x.imag.copy())
+        auto = lib.faf_ragged_block(r)
+        say(f"  automatic workgroup size {auto}")
+        assert auto in (256, 512)
+        if flags == 0:
+            assert auto == 256, "every C3 length runs on 256 threads"
+        shapes = [(0, 0), (1, 0), (0, 512), (1, 512), (0, 0)]
+        if auto == 256:
+            shapes.append((1, 256))
+        else:
+            assert lib.faf_ragged_set_block(r, 256) == -1
+            assert lib.faf_ragged_block(r) == 512
         outs = []
-        for persist in (0, 1, 0, 1):
+        for persist, block in shapes:
+            fa.check(lib.faf_ragged_set_block(r, block), "set block")
+            assert lib.faf_ragged_block(r) == (block or auto)
             fa.check(lib.faf_ragged_set_persistent(r, persist), "set persistent")
             yr, yi = G.empty(tot), G.empty(tot)
             for _ in range(3):
