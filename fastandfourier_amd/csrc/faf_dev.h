@@ -140,11 +140,17 @@ typedef struct fafd_ssig {
 } fafd_ssig;
 
 /* one unit of ragged work: `count` signals of one plan */
+/* one unit of the ragged launch; the path, thread count and first offset
+ * are copies of what the plan and the offset list hold, so a unit's data
+ * loads depend on this one record only */
 typedef struct fafd_unit {
     int32_t plan;
     int32_t count;
     int32_t first;       /* index into the sorted signal list */
+    int32_t path;        /* the plan's compiled path (-1: runtime passes) */
+    int32_t threads;     /* threads per signal */
     int32_t pad;
+    int64_t off0;        /* offset of the unit's first signal */
 } fafd_unit;
 
 #endif

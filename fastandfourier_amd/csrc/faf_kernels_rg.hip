@@ -94,25 +94,42 @@ template <class P> FAF_DEV P *wave_uniform(P *p) {
 }
 
 /* 8192 points: the single-pass register-resident FFT, 256 threads and
- * 35 KB of LDS per signal, so a 512-thread workgroup runs two signals.  A
- * signal's 256 threads are four whole waves, so its plane pointers are
- * wave-uniform: in scalar registers the 64 loads and stores address by a
- * 32-bit lane offset instead of a 64-bit address per load. */
+ * 35 KB of LDS per signal (two signals per 512-thread unit).  A signal's
+ * 256 threads are four whole waves, so its plane pointers and its active
+ * flag are wave-uniform.  The lane index is taken from threadIdx.x so that
+ * the compiler knows its range: with the pointers in scalar registers every
+ * load and store then addresses by a 32-bit lane offset plus an immediate.
+ * An inactive slot (a 512-thread unit holding one signal) reads the signal
+ * of slot 0, which its source points at, and stores nothing, so the loads
+ * need no per-load branch (with one the compiler formed a branch and a
+ * 64-bit address per load pair: 13% more VALU instructions than the
+ * batched kernel). */
 FAF_DEV void rg_rs8(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst, int sid,
-                    int lt, int active) {
-    float *lds = (float *)faf_smem + sid * (rs8::LDS_BYTES / 4);
+                    int, int active) {
+    const int lt = (int)threadIdx.x & (rs8::T - 1);
+    float *lds = (float *)faf_smem + __builtin_amdgcn_readfirstlane(sid) * (rs8::LDS_BYTES / 4);
     const float2 *tw = (const float2 *)pl->fwd.tw;
-    const float *ir = wave_uniform(src.re), *ii = wave_uniform(src.im);
-    float *orr = wave_uniform(dst.re), *oi = wave_uniform(dst.im);
+    /* global (not generic) pointers: loads and stores stay global_* */
+    typedef __attribute__((address_space(1))) float gf;
+    const gf *ir = (const gf *)wave_uniform(src.re), *ii = (const gf *)wave_uniform(src.im);
+    gf *orr = (gf *)wave_uniform(dst.re), *oi = (gf *)wave_uniform(dst.im);
+    const int act = __builtin_amdgcn_readfirstlane(active);
     float2 v[32], b1[4][3];
-#pragma unroll
-    for (int m = 0; m < 32; ++m) {
-        const int k = lt + rs8::T * m;
-        v[m] = active ? make_float2(ir[k], ii[k]) : make_float2(0.0f, 0.0f);
-    }
+    /* the twiddle bases first: issued after the data loads they keep more
+     * registers live (129 against 125 VGPRs, over the kernel's 128) */
     rs8::tw1_bases(tw, lt, b1);
+    if (act) {
+#pragma unroll
+        for (int m = 0; m < 32; ++m) {
+            const int k = lt + rs8::T * m;
+            v[m] = make_float2(ir[k], ii[k]);
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 32; ++m) v[m] = make_float2(0.0f, 0.0f);
+    }
     rs8::fft8192<false>(v, lds, tw, b1, lt);
-    if (active) {
+    if (act) {
 #pragma unroll
         for (int j = 0; j < 32; ++j) {
             const int k = lt + rs8::T * j;
@@ -160,9 +177,9 @@ FAF_DEV void rg_rs8(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &
 /* One unit: up to spb signals of one plan.  The switch is uniform across
  * the workgroup, so the barriers inside every path stay uniform. */
 template <bool GEN>
-FAF_DEV void rg_unit(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst, int sid,
-                     int lt, int active) {
-    switch (pl->path) {
+FAF_DEV void rg_unit(int path, const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst,
+                     int sid, int lt, int active) {
+    switch (path) {
 #define RG_CASE(id, kind, n, v, body) \
     case id: body(pl, src, dst, sid, lt, active); break;
         RG_PATHS(RG_CASE)
@@ -208,15 +225,19 @@ template <bool GEN>
 FAF_DEV void rg_run_unit(const fafd_rplan *plans, const fafd_unit *units, int u, const long *uoff,
                          const float *in_re, const float *in_im, float *out_re, float *out_im,
                          int tid) {
+    /* everything the data loads need comes from the unit record: a unit of
+     * one signal reads no offset list and no plan record before them */
     const fafd_unit un = units[u];
     const fafd_rplan *pl = plans + un.plan;
-    const int T = pl->threads;
+    const int T = un.threads;
     const int sid = tid / T, lt = tid - sid * T;
     const int active = sid < un.count;
-    const long off = uoff[un.first + (active ? sid : 0)];
+    long off;
+    if (un.count == 1) off = un.off0;
+    else off = uoff[un.first + (active ? sid : 0)];
     GSplitIn src{in_re + off, in_im + off};
     GSplitOut dst{out_re + off, out_im + off, 1.0f};
-    rg_unit<GEN>(pl, src, dst, sid, lt, active);
+    rg_unit<GEN>(un.path, pl, src, dst, sid, lt, active);
 }
 
 template <int MT, bool GEN, bool PERSIST>

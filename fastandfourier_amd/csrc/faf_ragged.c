@@ -92,9 +92,12 @@ static int cmp_size(const void *a, const void *b) {
 
 typedef struct { fafd_unit u; double cost; } unit_c;
 
+/* largest cost first; equal costs keep the signal order (units of one
+ * length then walk the signals in memory order) */
 static int cmp_cost(const void *a, const void *b) {
-    double x = ((const unit_c *)a)->cost, y = ((const unit_c *)b)->cost;
-    return x > y ? -1 : x < y;
+    const unit_c *p = a, *q = b;
+    if (p->cost != q->cost) return p->cost > q->cost ? -1 : 1;
+    return (p->u.first > q->u.first) - (p->u.first < q->u.first);
 }
 
 static void *up(const void *h, size_t bytes) {
@@ -368,7 +371,10 @@ static int ragged_build(faf_ragged *r, const rg_overrides *ov) {
             uc[nuu].u.plan = k;
             uc[nuu].u.count = c;
             uc[nuu].u.first = start[k] + s;
+            uc[nuu].u.path = tab[k].path;
+            uc[nuu].u.threads = tab[k].threads;
             uc[nuu].u.pad = 0;
+            uc[nuu].u.off0 = off[order[start[k] + s]];
             uc[nuu].cost = per_sig * c;
             ++nuu;
         }
