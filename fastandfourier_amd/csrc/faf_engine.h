@@ -26,6 +26,18 @@
 
 FAF_DEV float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 FAF_DEV float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+/* Twiddle loads: from a generic pointer, or from a pointer typed global.
+ * Kernels that read their table pointers out of device memory (the ragged
+ * launch's plan records) pass the second kind: a generic pointer of unknown
+ * origin becomes flat loads, which also hold up every LDS wait. */
+typedef float faf_f2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const faf_f2v faf_gtw;
+FAF_DEV float2 twld(const float2 *p, int i) { return p[i]; }
+FAF_DEV float2 twld(const faf_gtw *p, int i) {
+    const faf_f2v v = p[i];
+    return make_float2(v.x, v.y);
+}
+
 FAF_DEV float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }

@@ -96,9 +96,9 @@ struct CtChain<INV, N, T, NS, R, Rest...> {
         }
     }
 
-    template <class S, class D>
+    template <class S, class D, class TP>
     FAF_DEV void stage_lazy(const S &src, const D &dst, int lt, int active, bool sync,
-                            const float2 *__restrict__ tw) {
+                            TP __restrict__ tw) {
         float2 v[B][R];
         float2 tl[NS > 1 ? B : 1][NS > 1 ? K : 1];
 #pragma unroll
@@ -110,7 +110,7 @@ struct CtChain<INV, N, T, NS, R, Rest...> {
                 if constexpr (NS > 1) {
                     const int step = (j % NS) * (N / (NS * R));
 #pragma unroll
-                    for (int i = 0; i < K; ++i) tl[b][i] = tw[step << i];
+                    for (int i = 0; i < K; ++i) tl[b][i] = twld(tw, step << i);
                 }
             }
         }
@@ -134,9 +134,9 @@ struct CtChain<INV, N, T, NS, R, Rest...> {
         }
     }
 
-    template <class S, class D>
+    template <class S, class D, class TP>
     FAF_DEV void run_lazy(const S &src, const Lds &lds, const D &dst, int lt, int active,
-                          const float2 *tw) {
+                          TP tw) {
         if constexpr (sizeof...(Rest) == 0) {
             stage_lazy(src, dst, lt, active, S::is_lds || (!S::is_lds && !D::is_lds), tw);
         } else {

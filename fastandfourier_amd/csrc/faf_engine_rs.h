@@ -159,12 +159,12 @@ constexpr int Q = 33;
 constexpr int LDS_BYTES = 8 * P2 * 4;
 
 /* twiddle bases of pass 1: W^(n1 2^e), e < 3, for the four n1 of a thread */
-FAF_DEV void tw1_bases(const float2 *tw, int tid, float2 (*b)[3]) {
+template <class TP> FAF_DEV void tw1_bases(TP tw, int tid, float2 (*b)[3]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int n1 = tid + 256 * i;
 #pragma unroll
-        for (int e = 0; e < 3; ++e) b[i][e] = tw[n1 << e];
+        for (int e = 0; e < 3; ++e) b[i][e] = twld(tw, n1 << e);
     }
 }
 
@@ -191,8 +191,8 @@ template <class WO, class RO> FAF_DEV void xchg(float2 *v, float *lds, int wb, i
 
 /* In: v[m] = z[tid + 256 m], b1 from tw1_bases.  Out: v[j] = Z[tid + 256 j].
  * Ends with LDS reads outstanding in other waves: barrier before reusing lds. */
-template <bool INV>
-FAF_DEV void fft8192(float2 *v, float *lds, const float2 *tw, const float2 (*b1)[3], int tid) {
+template <bool INV, class TP>
+FAF_DEV void fft8192(float2 *v, float *lds, TP tw, const float2 (*b1)[3], int tid) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         float2 g[8];
@@ -212,7 +212,7 @@ FAF_DEV void fft8192(float2 *v, float *lds, const float2 *tw, const float2 (*b1)
     const int k2 = tid & 7, n1a = tid >> 3;
     float2 b2[5];
 #pragma unroll
-    for (int e = 0; e < 5; ++e) b2[e] = tw[(8 * n1a) << e];
+    for (int e = 0; e < 5; ++e) b2[e] = twld(tw, (8 * n1a) << e);
     xchg<WoY, RoY>(v, lds, tid, k2 * P1 + n1a);
     rs::dft32<INV>(v);
     rs::tw_apply(v, b2);

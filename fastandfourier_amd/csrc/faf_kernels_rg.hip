@@ -32,8 +32,43 @@ __global__ void kg_fill_u32(unsigned *p, int n, unsigned v) {
     if (i < n) p[i] = v;
 }
 
+/* Table pointers (twiddles, Rader index and chirp tables) come out of the
+ * plan records in device memory, so the compiler cannot tell their address
+ * space: as generic pointers their loads are flat loads, which also count
+ * against the LDS wait counter (every LDS wait of a pass then waits for the
+ * pass's twiddles too).  Every table is device memory: these give the loads
+ * global pointers. */
+FAF_DEV const faf_gtw *gtw(const void *p) { return (const faf_gtw *)p; }
+typedef __attribute__((address_space(1))) const int32_t faf_gi32;
+
+struct GatherLdsG {          /* a[j] = x[rin[j]] from the staged signal */
+    Lds x;
+    const faf_gi32 *rin;
+    static constexpr bool is_lds = true;
+    FAF_DEV float2 ld(int j) const { return x.ld(rin[j]); }
+};
+
+struct MulLdsG {             /* A[j] * B[j] */
+    Lds a;
+    const faf_gtw *b;
+    static constexpr bool is_lds = true;
+    FAF_DEV float2 ld(int j) const { return cmul(a.ld(j), twld(b, j)); }
+};
+
+struct ScatterLdsG {         /* y[rout[j]] = x0 + v  (B carries the 1/m) */
+    Lds y;
+    const faf_gi32 *rout;
+    float2 x0;
+    static constexpr bool is_lds = true;
+    FAF_DEV void st(int j, float2 v) const { y.st(rout[j], cadd(x0, v)); }
+};
+
 /* Passes issue their twiddle loads lazily (CtChain::run_lazy): the kernel's
- * occupancy is set by its register count, not by latency inside one pass. */
+ * occupancy is set by its register count, not by latency inside one pass.
+ * Only the Rader paths read their tables through global pointers (1009
+ * alone: 18.6 -> 17.3 us, the C3 mix 1-2% faster); the 5-smooth chains and
+ * the 8192 pass keep generic twiddle pointers, as global ones measured no
+ * faster there. */
 template <int N>
 FAF_DEV void rg_ct(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst, int sid,
                    int lt, int active) {
@@ -68,15 +103,15 @@ FAF_DEV void rg_rader(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut
         for (int i = lt; i < P; i += T) aa.st(i, src.ld(i));
     __syncthreads();
     const float2 x0 = active ? aa.ld(0) : make_float2(0.0f, 0.0f);
-    GatherLds g{aa, R.rin};
+    GatherLdsG g{aa, (const faf_gi32 *)R.rin};
     typename MixFFT<false, P - 1, T, Seq>::Chain fw;
-    fw.run_lazy(g, aa, aa, lt, active, (const float2 *)pl->fwd.tw);
+    fw.run_lazy(g, aa, aa, lt, active, gtw(pl->fwd.tw));
     __syncthreads();
     if (active && lt == 0) dst.st(0, cadd(x0, aa.ld(0)));   /* DC = x0 + sum x[1..] */
-    MulLds mb{aa, (const float2 *)R.b};
-    ScatterLds sc{aa, R.rout, x0};
+    MulLdsG mb{aa, gtw(R.b)};
+    ScatterLdsG sc{aa, (const faf_gi32 *)R.rout, x0};
     typename MixFFT<true, P - 1, T, Seq>::Chain bw;
-    bw.run_lazy(mb, aa, sc, lt, active, (const float2 *)pl->inv.tw);
+    bw.run_lazy(mb, aa, sc, lt, active, gtw(pl->inv.tw));
     __syncthreads();
     if (active)
         for (int i = 1 + lt; i < P; i += T) dst.st(i, aa.ld(i));
