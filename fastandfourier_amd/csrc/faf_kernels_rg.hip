@@ -63,6 +63,35 @@ struct ScatterLdsG {         /* y[rout[j]] = x0 + v  (B carries the 1/m) */
     FAF_DEV void st(int j, float2 v) const { y.st(rout[j], cadd(x0, v)); }
 };
 
+/* A unit's signals are addressed from its first signal: one base pointer per
+ * plane, the same for the whole workgroup (scalar registers), and a per-lane
+ * offset d in elements of the lane's signal from that base.  The host keeps
+ * every unit within 2^30 elements of its first signal, so the byte offsets
+ * fit 32 bits and each load and store takes the scalar-base form with a
+ * 32-bit lane offset (a per-lane 64-bit pointer costs a 64-bit add per
+ * access group). */
+struct RgIn {
+    const float *re, *im;
+    uint32_t d;
+    static constexpr bool is_lds = false;
+    FAF_DEV float2 ld(int i) const {
+        const uint32_t b = (d + (uint32_t)i) * 4u;
+        return make_float2(*(const float *)((const char *)re + b),
+                           *(const float *)((const char *)im + b));
+    }
+};
+
+struct RgOut {
+    float *re, *im;
+    uint32_t d;
+    static constexpr bool is_lds = false;
+    FAF_DEV void st(int i, float2 v) const {
+        const uint32_t b = (d + (uint32_t)i) * 4u;
+        *(float *)((char *)re + b) = v.x;
+        *(float *)((char *)im + b) = v.y;
+    }
+};
+
 /* Passes issue their twiddle loads lazily (CtChain::run_lazy): the kernel's
  * occupancy is set by its register count, not by latency inside one pass.
  * Only the Rader paths read their tables through global pointers (1009
@@ -70,7 +99,7 @@ struct ScatterLdsG {         /* y[rout[j]] = x0 + v  (B carries the 1/m) */
  * the 8192 pass keep generic twiddle pointers, as global ones measured no
  * faster there. */
 template <int N>
-FAF_DEV void rg_ct(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst, int sid,
+FAF_DEV void rg_ct(const fafd_rplan *pl, const RgIn &src, const RgOut &dst, int sid,
                    int lt, int active) {
     Lds lds{faf_smem + sid * pl->fwd.plane};
     typename Ct<false, N>::Chain ch;
@@ -78,7 +107,7 @@ FAF_DEV void rg_ct(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &d
 }
 
 template <int N, int V>
-FAF_DEV void rg_mix(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst, int sid,
+FAF_DEV void rg_mix(const fafd_rplan *pl, const RgIn &src, const RgOut &dst, int sid,
                     int lt, int active) {
     Lds lds{faf_smem + sid * pl->fwd.plane};
     typename MixFFT<false, N, MixSeq<N, V>::T, typename MixSeq<N, V>::type>::Chain ch;
@@ -93,7 +122,7 @@ FAF_DEV void rg_mix(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &
  * scatters y[rout[j]] back into it, again after its barrier; y[0] goes
  * straight to the output. */
 template <int P, int V>
-FAF_DEV void rg_rader(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst,
+FAF_DEV void rg_rader(const fafd_rplan *pl, const RgIn &src, const RgOut &dst,
                       int sid, int lt, int active) {
     constexpr int T = RaderSeq<P, V>::T;
     using Seq = typename RaderSeq<P, V>::type;
@@ -117,17 +146,6 @@ FAF_DEV void rg_rader(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut
         for (int i = 1 + lt; i < P; i += T) dst.st(i, aa.ld(i));
 }
 
-/* GEN: also the runtime Stockham, Rader and Bluestein paths.  Without them
- * the kernel needs far fewer registers (more workgroups per CU); the host
- * picks it when every length of the batch has compile-time passes. */
-/* a pointer that is the same in every lane of the wave, in scalar registers */
-template <class P> FAF_DEV P *wave_uniform(P *p) {
-    const uint64_t v = (uint64_t)p;
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return (P *)(((uint64_t)hi << 32) | lo);
-}
-
 /* 8192 points: the single-pass register-resident FFT, 256 threads and
  * 35 KB of LDS per signal (two signals per 512-thread unit).  A signal's
  * 256 threads are four whole waves, so its plane pointers and its active
@@ -139,15 +157,16 @@ template <class P> FAF_DEV P *wave_uniform(P *p) {
  * need no per-load branch (with one the compiler formed a branch and a
  * 64-bit address per load pair: 13% more VALU instructions than the
  * batched kernel). */
-FAF_DEV void rg_rs8(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst, int sid,
+FAF_DEV void rg_rs8(const fafd_rplan *pl, const RgIn &src, const RgOut &dst, int sid,
                     int, int active) {
     const int lt = (int)threadIdx.x & (rs8::T - 1);
     float *lds = (float *)faf_smem + __builtin_amdgcn_readfirstlane(sid) * (rs8::LDS_BYTES / 4);
     const float2 *tw = (const float2 *)pl->fwd.tw;
     /* global (not generic) pointers: loads and stores stay global_* */
     typedef __attribute__((address_space(1))) float gf;
-    const gf *ir = (const gf *)wave_uniform(src.re), *ii = (const gf *)wave_uniform(src.im);
-    gf *orr = (gf *)wave_uniform(dst.re), *oi = (gf *)wave_uniform(dst.im);
+    const uint32_t du = __builtin_amdgcn_readfirstlane(src.d);   /* whole waves per slot */
+    const gf *ir = (const gf *)(src.re + du), *ii = (const gf *)(src.im + du);
+    gf *orr = (gf *)(dst.re + du), *oi = (gf *)(dst.im + du);
     const int act = __builtin_amdgcn_readfirstlane(active);
     float2 v[32], b1[4][3];
     /* the twiddle bases first: issued after the data loads they keep more
@@ -210,9 +229,12 @@ FAF_DEV void rg_rs8(const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &
     X(25, FAFD_KIND_RADER_CT, 1009, 1, (rg_rader<1009, 1>))
 
 /* One unit: up to spb signals of one plan.  The switch is uniform across
- * the workgroup, so the barriers inside every path stay uniform. */
+ * the workgroup, so the barriers inside every path stay uniform.  GEN: also
+ * the runtime Stockham, Rader and Bluestein paths.  Without them the kernel
+ * needs far fewer registers (more workgroups per CU); the host picks it
+ * when every length of the batch has compile-time passes. */
 template <bool GEN>
-FAF_DEV void rg_unit(int path, const fafd_rplan *pl, const GSplitIn &src, const GSplitOut &dst,
+FAF_DEV void rg_unit(int path, const fafd_rplan *pl, const RgIn &src, const RgOut &dst,
                      int sid, int lt, int active) {
     switch (path) {
 #define RG_CASE(id, kind, n, v, body) \
@@ -267,11 +289,10 @@ FAF_DEV void rg_run_unit(const fafd_rplan *plans, const fafd_unit *units, int u,
     const int T = un.threads;
     const int sid = tid / T, lt = tid - sid * T;
     const int active = sid < un.count;
-    long off;
-    if (un.count == 1) off = un.off0;
-    else off = uoff[un.first + (active ? sid : 0)];
-    GSplitIn src{in_re + off, in_im + off};
-    GSplitOut dst{out_re + off, out_im + off, 1.0f};
+    uint32_t d = 0;
+    if (un.count > 1) d = (uint32_t)(uoff[un.first + (active ? sid : 0)] - un.off0);
+    RgIn src{in_re + un.off0, in_im + un.off0, d};
+    RgOut dst{out_re + un.off0, out_im + un.off0, d};
     rg_unit<GEN>(un.path, pl, src, dst, sid, lt, active);
 }
 

@@ -30,6 +30,8 @@
  * faster, the C3 mix 170 -> 154 us), else 512 */
 #define RAGGED_BLOCK 512
 #define RAGGED_BLOCK_SMALL 256
+/* a unit's signals lie within this many elements of its first */
+#define RAGGED_SPAN (1L << 30)
 #define RAGGED_LDS_CAP (96 * 1024)
 /* with compile-time paths only, two workgroups share a CU's 160 KB */
 #define RAGGED_LDS_CAP_CT (80 * 1024 - 256)
@@ -264,6 +266,15 @@ static int ragged_per_cu(int generic, size_t lds, int block) {
     return n < 1 ? 1 : n;
 }
 
+/* Signals in the next unit of one length: up to spb of the `left` signals
+ * at sig[0..], as long as each ends within RAGGED_SPAN elements of the
+ * first one's offset. */
+static int unit_len(const long *off, const int32_t *sig, int left, int spb, size_t len) {
+    int c = left < spb ? left : spb;
+    while (c > 1 && off[sig[c - 1]] + (long)len - off[sig[0]] > RAGGED_SPAN) --c;
+    return c;
+}
+
 /* Builds the device tables of a ragged batch into r (r->lengths/count set). */
 static int ragged_build(faf_ragged *r, const rg_overrides *ov) {
     const size_t count = r->count;
@@ -356,9 +367,14 @@ static int ragged_build(faf_ragged *r, const rg_overrides *ov) {
     int *fill = calloc((size_t)nu, sizeof(int));
     for (size_t i = 0; i < count; ++i) order[start[pid[i]] + fill[pid[i]]++] = (int32_t)i;
 
-    /* units, largest estimated cost first */
+    /* units, largest estimated cost first.  A unit's signals lie within
+     * RAGGED_SPAN elements of its first (the kernel addresses them by 32-bit
+     * byte offsets from it), so a unit ends early where the next signal of
+     * its length lies further away. */
     int maxu = 0;
-    for (int k = 0; k < nu; ++k) maxu += (nper[k] + tab[k].spb - 1) / tab[k].spb;
+    for (int k = 0; k < nu; ++k)
+        for (int s = 0; s < nper[k]; ++maxu)
+            s += unit_len(off, order + start[k] + s, nper[k] - s, tab[k].spb, uniq[k]);
     unit_c *uc = malloc((size_t)maxu * sizeof(unit_c));
     int nuu = 0;
     for (int k = 0; k < nu; ++k) {
@@ -366,17 +382,19 @@ static int ragged_build(faf_ragged *r, const rg_overrides *ov) {
         for (size_t m = uniq[k]; m > 1; m >>= 1) lg += 1.0;
         const int stockham = tab[k].kind == FAFD_KIND_STOCKHAM || tab[k].kind == FAFD_KIND_CT;
         double per_sig = (double)uniq[k] * lg * (stockham ? 1.0 : 2.5);
-        for (int s = 0; s < nper[k]; s += tab[k].spb) {
-            int c = nper[k] - s < tab[k].spb ? nper[k] - s : tab[k].spb;
+        for (int s = 0; s < nper[k];) {
+            const long o0 = off[order[start[k] + s]];
+            const int c = unit_len(off, order + start[k] + s, nper[k] - s, tab[k].spb, uniq[k]);
             uc[nuu].u.plan = k;
             uc[nuu].u.count = c;
             uc[nuu].u.first = start[k] + s;
             uc[nuu].u.path = tab[k].path;
             uc[nuu].u.threads = tab[k].threads;
             uc[nuu].u.pad = 0;
-            uc[nuu].u.off0 = off[order[start[k] + s]];
+            uc[nuu].u.off0 = o0;
             uc[nuu].cost = per_sig * c;
             ++nuu;
+            s += c;
         }
     }
     qsort(uc, (size_t)nuu, sizeof(unit_c), cmp_cost);
