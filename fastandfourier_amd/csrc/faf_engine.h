@@ -32,6 +32,9 @@ FAF_DEV float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.
  * origin becomes flat loads, which also hold up every LDS wait. */
 typedef float faf_f2v __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) const faf_f2v faf_gtw;
+typedef __attribute__((address_space(1))) faf_f2v faf_gf2;
+typedef __attribute__((address_space(1))) const float faf_gcf;
+typedef __attribute__((address_space(1))) const int32_t faf_gi32;
 FAF_DEV float2 twld(const float2 *p, int i) { return p[i]; }
 FAF_DEV float2 twld(const faf_gtw *p, int i) {
     const faf_f2v v = p[i];
@@ -377,7 +380,7 @@ template <int R> FAF_DEV void tw_expand(const float2 *b, float2 *w) {
 
 template <int R, bool INV, class S, class D>
 FAF_DEV void stage(const S &src, const D &dst, int n, int active, int lt, int T,
-                   const fafd_stage &st, const float2 *__restrict__ tw, bool sync) {
+                   const fafd_stage &st, const faf_gtw *__restrict__ tw, bool sync) {
     constexpr int B = FAFD_MAXE / R, K = TwBits<R>::K;
     const int nb = n / R;
     const int nbe = active ? nb : 0;
@@ -393,7 +396,7 @@ FAF_DEV void stage(const S &src, const D &dst, int n, int active, int lt, int T,
                 const int k = j - (int)__umulhi((unsigned)j, st.ns_magic) * st.ns;
                 const int step = k * st.tw_stride;
 #pragma unroll
-                for (int i = 0; i < K; ++i) tb[b][i] = tw[step << i];
+                for (int i = 0; i < K; ++i) tb[b][i] = twld(tw, step << i);
             }
         }
     }
@@ -420,7 +423,7 @@ FAF_DEV void stage(const S &src, const D &dst, int n, int active, int lt, int T,
 
 template <bool INV, class S, class D>
 FAF_DEV void stage_any(const S &src, const D &dst, int n, int active, int lt, int T,
-                       const fafd_stage &st, const float2 *tw, bool sync) {
+                       const fafd_stage &st, const faf_gtw *tw, bool sync) {
     switch (st.radix) {
     case 2: stage<2, INV>(src, dst, n, active, lt, T, st, tw, sync); break;
     case 3: stage<3, INV>(src, dst, n, active, lt, T, st, tw, sync); break;
@@ -441,7 +444,9 @@ FAF_DEV void stage_any(const S &src, const D &dst, int n, int active, int lt, in
 template <bool INV, class S, class D>
 FAF_DEV void fft_run(const fafd_fft &P, const S &src, const Lds &lds, const D &dst,
                      int lt, int active) {
-    const float2 *tw = (const float2 *)P.tw;
+    /* the plan may live in device memory (its pointers then have no known
+     * address space): the table is device memory, read by global loads */
+    const faf_gtw *tw = (const faf_gtw *)P.tw;
     const int ns = P.nstages, n = P.n, T = P.threads;
     if (ns == 1) {
         stage_any<INV>(src, dst, n, active, lt, T, P.st[0], tw, true);
@@ -1011,23 +1016,26 @@ __global__ void k_c2r_pre(int nc, const float2 *pre, const float *in_re, const f
 /* (reference src/faf_bluestein.c:165-223) inside one workgroup         */
 /* ------------------------------------------------------------------ */
 
+/* The tables of Rader and Bluestein plans are device memory; typed global,
+ * their reads stay global loads when the plan itself was read from device
+ * memory (serving and ragged launches), instead of flat loads. */
 struct GatherLds {           /* a[j] = x[rin[j]] from the staged signal */
     Lds x;
-    const int32_t *rin;
+    const faf_gi32 *rin;
     static constexpr bool is_lds = true;
     FAF_DEV float2 ld(int j) const { return x.ld(rin[j]); }
 };
 
 struct MulLds {              /* A[j] * B[j] */
     Lds a;
-    const float2 *b;
+    const faf_gtw *b;
     static constexpr bool is_lds = true;
-    FAF_DEV float2 ld(int j) const { return cmul(a.ld(j), b[j]); }
+    FAF_DEV float2 ld(int j) const { return cmul(a.ld(j), twld(b, j)); }
 };
 
 struct ScatterLds {          /* y[rout[j]] = x0 + v  (B carries the 1/m) */
     Lds y;
-    const int32_t *rout;
+    const faf_gi32 *rout;
     float2 x0;
     static constexpr bool is_lds = true;
     FAF_DEV void st(int j, float2 v) const { y.st(rout[j], cadd(x0, v)); }
@@ -1044,12 +1052,12 @@ FAF_DEV void rader_body(const fafd_fft &Pf, const fafd_fft &Pi, const fafd_rader
         for (int i = lt; i < p; i += T) xa.st(i, src.ld(i));
     __syncthreads();
     float2 x0 = active ? xa.ld(0) : make_float2(0.0f, 0.0f);
-    GatherLds g{xa, R.rin};
+    GatherLds g{xa, (const faf_gi32 *)R.rin};
     fft_run<false>(Pf, g, aa, aa, lt, active);
     __syncthreads();
     if (active && lt == 0) xa.st(0, cadd(x0, aa.ld(0)));   /* DC = x0 + sum x[1..] */
-    MulLds mb{aa, (const float2 *)R.b};
-    ScatterLds sc{xa, R.rout, x0};
+    MulLds mb{aa, (const faf_gtw *)R.b};
+    ScatterLds sc{xa, (const faf_gi32 *)R.rout, x0};
     fft_run<true>(Pi, mb, aa, sc, lt, active);
     __syncthreads();
     if (active)
@@ -1058,32 +1066,32 @@ FAF_DEV void rader_body(const fafd_fft &Pf, const fafd_fft &Pi, const fafd_rader
 
 template <class S> struct ChirpIn {   /* a[k] = x[k] chirp[k] for k < n, 0 up to m */
     S x;
-    const float2 *chirp;
+    const faf_gtw *chirp;
     int n;
     static constexpr bool is_lds = false;
     FAF_DEV float2 ld(int k) const {
-        return k < n ? cmul(x.ld(k), chirp[k]) : make_float2(0.0f, 0.0f);
+        return k < n ? cmul(x.ld(k), twld(chirp, k)) : make_float2(0.0f, 0.0f);
     }
 };
 
 template <class D> struct ChirpOut {
     D out;
-    const float2 *chirp;
+    const faf_gtw *chirp;
     int n;
     static constexpr bool is_lds = false;
     FAF_DEV void st(int k, float2 v) const {
-        if (k < n) out.st(k, cmul(v, chirp[k]));
+        if (k < n) out.st(k, cmul(v, twld(chirp, k)));
     }
 };
 
 template <class S, class D>
 FAF_DEV void blue_body(const fafd_fft &Pf, const fafd_fft &Pi, const fafd_blue &Bq,
                        const S &src, const D &dst, Lds aa, int lt, int active) {
-    ChirpIn<S> ci{src, (const float2 *)Bq.chirp, Bq.n};
+    ChirpIn<S> ci{src, (const faf_gtw *)Bq.chirp, Bq.n};
     fft_run<false>(Pf, ci, aa, aa, lt, active);
     __syncthreads();
-    MulLds mb{aa, (const float2 *)Bq.b};
-    ChirpOut<D> co{dst, (const float2 *)Bq.chirp, Bq.n};
+    MulLds mb{aa, (const faf_gtw *)Bq.b};
+    ChirpOut<D> co{dst, (const faf_gtw *)Bq.chirp, Bq.n};
     fft_run<true>(Pi, mb, aa, co, lt, active);
 }
 
@@ -1160,9 +1168,65 @@ __global__ void k_fill_u32(unsigned *p, int n, unsigned v) {
 /* up to `count` same-plan signals with their own in/out pointers        */
 /* ------------------------------------------------------------------ */
 
+/* The serving kernel reads its plans and request records from device
+ * memory, so every pointer it holds is of unknown address space to the
+ * compiler and its accesses would be flat (each also counts against the LDS
+ * wait counter).  Its inputs, results, workspaces and tables are all device
+ * memory: these accessors type them global. */
+FAF_DEV void gst2(faf_gf2 *p, long i, float2 v) {
+    faf_f2v r = {v.x, v.y};
+    p[i] = r;
+}
+
+struct SvInterIn {           /* interleaved complex */
+    const faf_gtw *p;
+    static constexpr bool is_lds = false;
+    FAF_DEV float2 ld(int i) const { return twld(p, i); }
+};
+
+struct SvInterOut {
+    faf_gf2 *p;
+    float s;
+    static constexpr bool is_lds = false;
+    FAF_DEV void st(int i, float2 v) const { gst2(p, i, cscale(v, s)); }
+};
+
+struct SvPairIn {            /* real samples as pairs z[i] = x[2i] + i x[2i+1] */
+    const faf_gcf *x;
+    static constexpr bool is_lds = false;
+    FAF_DEV float2 ld(int i) const { return make_float2(x[2 * i], x[2 * i + 1]); }
+};
+
+struct SvFrameIn {           /* windowed frame, zero past the record (as GFrameIn) */
+    const faf_gcf *x;
+    const faf_gcf *w;
+    long avail;
+    static constexpr bool is_lds = false;
+    FAF_DEV float2 ld(int i) const {
+        const long a = 2L * i, b = a + 1;
+        if (b < avail) return make_float2(x[a] * w[a], x[b] * w[b]);
+        float u = a < avail ? x[a] * w[a] : 0.0f;
+        return make_float2(u, 0.0f);
+    }
+};
+
+struct SvOneSidedIn {        /* as OneSidedIn, with the filter row typed global */
+    Lds X;
+    const faf_gcf *psi;
+    int nc;
+    static constexpr bool is_lds = true;
+    FAF_DEV float2 ld(int k) const {
+        if (k > nc) return make_float2(0.0f, 0.0f);
+        float2 v = X.ld(k);
+        const float p = psi[k];
+        if (k == 0 || k == nc) return make_float2(v.x * p, 0.0f);
+        return make_float2(2.0f * v.x * p, 2.0f * v.y * p);
+    }
+};
+
 /* Kiss post-pass from LDS Z into an interleaved Hermitian row */
-FAF_DEV void post_pass_inter(const Lds &z, const float2 *post, int nc, int lt, int T,
-                             float2 *out) {
+FAF_DEV void post_pass_inter(const Lds &z, const faf_gtw *post, int nc, int lt, int T,
+                             faf_gf2 *out) {
     for (int k = lt; k <= nc / 2; k += T) {
         const int m = nc - k;
         float2 xk, xm;
@@ -1171,10 +1235,10 @@ FAF_DEV void post_pass_inter(const Lds &z, const float2 *post, int nc, int lt, i
             xk = make_float2(v.x + v.y, 0.0f);
             xm = make_float2(v.x - v.y, 0.0f);
         } else {
-            kiss_post(z.ld(k), z.ld(m), post[k], xk, xm);
+            kiss_post(z.ld(k), z.ld(m), twld(post, k), xk, xm);
         }
-        out[k] = xk;
-        out[m] = xm;
+        gst2(out, k, xk);
+        gst2(out, m, xm);
     }
 }
 
@@ -1185,20 +1249,21 @@ FAF_DEV void serve_phase_unit(const fafd_splan *pl, const fafd_ssig &sg, int tid
     if (pl->kind == FAFD_SK_POST) {
         const int nc = pl->fs_other;
         const int k1 = sg.r0 + pl->fs_per < nc / 2 + 1 ? sg.r0 + pl->fs_per : nc / 2 + 1;
-        const float2 *z = (const float2 *)sg.in;
-        float2 *out = (float2 *)sg.out;
+        const faf_gtw *z = (const faf_gtw *)sg.in;
+        faf_gf2 *out = (faf_gf2 *)sg.out;
+        const faf_gtw *post = (const faf_gtw *)pl->post;
         for (int k = sg.r0 + tid; k < k1; k += bd) {
             const int m = nc - k;
             float2 xk, xm;
             if (k == 0) {
-                const float2 v = z[0];
+                const float2 v = twld(z, 0);
                 xk = make_float2(v.x + v.y, 0.0f);
                 xm = make_float2(v.x - v.y, 0.0f);
             } else {
-                kiss_post(z[k], z[m], ((const float2 *)pl->post)[k], xk, xm);
+                kiss_post(twld(z, k), twld(z, m), twld(post, k), xk, xm);
             }
-            out[k] = xk;
-            out[m] = xm;
+            gst2(out, k, xk);
+            gst2(out, m, xm);
         }
         return;
     }
@@ -1208,21 +1273,22 @@ FAF_DEV void serve_phase_unit(const fafd_splan *pl, const fafd_ssig &sg, int tid
     if (pl->kind == FAFD_SK_FSCOL) {
         /* columns c0 .. c0+per of x viewed as [n1][n2]: element (k1, c) */
         const int n2 = other, c0 = sg.r0;
-        const float2 *xc = (const float2 *)sg.in;
-        const float *psi = pl->psi + sg.avail * pl->psi_stride;
+        const faf_gtw *xc = (const faf_gtw *)sg.in;
+        const faf_gcf *xin = (const faf_gcf *)sg.in;
+        const faf_gcf *psi = (const faf_gcf *)pl->psi + sg.avail * pl->psi_stride;
         for (int f = tid; f < per * n; f += bd) {
             const int c = f % per, r = f / per;
             const long i = (long)r * n2 + c0 + c;
             float2 v = make_float2(0.0f, 0.0f);
             if (c0 + c < n2) {
                 if (pl->fs_in == FAFD_FSIN_INTER) {
-                    v = xc[i];
+                    v = twld(xc, i);
                 } else if (pl->fs_in == FAFD_FSIN_PAIR) {
-                    v = make_float2(sg.in[2 * i], sg.in[2 * i + 1]);
+                    v = make_float2(xin[2 * i], xin[2 * i + 1]);
                 } else if (i <= pl->fs_nc) {   /* analytic row spectrum X psi_r */
                     float w = psi[i];
                     if (i != 0 && i != pl->fs_nc) w *= 2.0f;
-                    v = cscale(xc[i], w);
+                    v = cscale(twld(xc, i), w);
                 }
             }
             Lds{faf_smem + c * P.plane}.st(r, v);
@@ -1232,23 +1298,23 @@ FAF_DEV void serve_phase_unit(const fafd_splan *pl, const fafd_ssig &sg, int tid
         if (pl->inv) fft_run<true>(P, lds, lds, lds, lt, sid < per);
         else fft_run<false>(P, lds, lds, lds, lt, sid < per);
         __syncthreads();
-        const float2 *tw = (const float2 *)pl->twn;
-        float2 *ws = (float2 *)sg.out;
+        const faf_gtw *tw = (const faf_gtw *)pl->twn;
+        faf_gf2 *ws = (faf_gf2 *)sg.out;
         for (int f = tid; f < per * n; f += bd) {
             const int c = f % per, k1 = f / per;
             if (c0 + c >= n2) continue;
             const float2 v = Lds{faf_smem + c * P.plane}.ld(k1);
-            ws[(long)k1 * n2 + c0 + c] = cmul(v, tw[(long)k1 * (c0 + c)]);
+            gst2(ws, (long)k1 * n2 + c0 + c, cmul(v, twld(tw + (long)k1 * (c0 + c), 0)));
         }
         return;
     }
     /* FSROW: rows k0 .. k0+per of ws [n1][n2], FFT n2, out[k1 + n1 k2] */
     const int n1 = other, k0 = sg.r0;
-    const float2 *ws = (const float2 *)sg.in;
+    const faf_gtw *ws = (const faf_gtw *)sg.in;
     for (int r = 0; r < per && k0 + r < n1; ++r) {
         Lds row{faf_smem + r * P.plane};
-        const long b = (long)(k0 + r) * n;
-        for (int c = tid; c < n; c += bd) row.st(c, ws[b + c]);
+        const faf_gtw *wr = ws + (long)(k0 + r) * n;
+        for (int c = tid; c < n; c += bd) row.st(c, twld(wr, c));
     }
     __syncthreads();
     Lds lds{faf_smem + sid * P.plane};
@@ -1256,11 +1322,12 @@ FAF_DEV void serve_phase_unit(const fafd_splan *pl, const fafd_ssig &sg, int tid
     if (pl->inv) fft_run<true>(P, lds, lds, lds, lt, act);
     else fft_run<false>(P, lds, lds, lds, lt, act);
     __syncthreads();
-    float2 *out = (float2 *)sg.out;
+    faf_gf2 *out = (faf_gf2 *)sg.out;
     for (int f = tid; f < per * n; f += bd) {
         const int r = f % per, k2 = f / per;
         if (k0 + r >= n1) continue;
-        out[(long)(k0 + r) + (long)n1 * k2] = cscale(Lds{faf_smem + r * P.plane}.ld(k2), pl->scale);
+        gst2(out, (long)(k0 + r) + (long)n1 * k2,
+             cscale(Lds{faf_smem + r * P.plane}.ld(k2), pl->scale));
     }
 }
 
@@ -1294,33 +1361,35 @@ __global__ __launch_bounds__(MT) void k_serve(const fafd_splan *plans, const faf
         switch (pl->kind) {
         case FAFD_SK_C2C: {
             Lds lds{base};
-            fft_run<false>(pl->a, GInterIn{(const float2 *)sg.in}, lds,
-                           GInterOut{(float2 *)sg.out, 1.0f}, lt, active);
+            fft_run<false>(pl->a, SvInterIn{(const faf_gtw *)sg.in}, lds,
+                           SvInterOut{(faf_gf2 *)sg.out, 1.0f}, lt, active);
             break;
         }
         case FAFD_SK_RADER: {
             Lds xa{base};
             Lds aa{base + lpad(pl->rader.p) + 1};
-            rader_body(pl->a, pl->b, pl->rader, GInterIn{(const float2 *)sg.in},
-                       GInterOut{(float2 *)sg.out, 1.0f}, xa, aa, lt, T, active);
+            rader_body(pl->a, pl->b, pl->rader, SvInterIn{(const faf_gtw *)sg.in},
+                       SvInterOut{(faf_gf2 *)sg.out, 1.0f}, xa, aa, lt, T, active);
             break;
         }
         case FAFD_SK_BLUE: {
             Lds aa{base};
-            blue_body(pl->a, pl->b, pl->blue, GInterIn{(const float2 *)sg.in},
-                      GInterOut{(float2 *)sg.out, 1.0f}, aa, lt, active);
+            blue_body(pl->a, pl->b, pl->blue, SvInterIn{(const faf_gtw *)sg.in},
+                      SvInterOut{(faf_gf2 *)sg.out, 1.0f}, aa, lt, active);
             break;
         }
         case FAFD_SK_R2C:
         case FAFD_SK_STFT: {
             Lds lds{base};
             if (pl->kind == FAFD_SK_R2C)
-                fft_run<false>(pl->a, GPairIn{sg.in}, lds, lds, lt, active);
+                fft_run<false>(pl->a, SvPairIn{(const faf_gcf *)sg.in}, lds, lds, lt, active);
             else
-                fft_run<false>(pl->a, GFrameIn{sg.in, pl->win, sg.avail}, lds, lds, lt, active);
+                fft_run<false>(pl->a,
+                               SvFrameIn{(const faf_gcf *)sg.in, (const faf_gcf *)pl->win, sg.avail},
+                               lds, lds, lt, active);
             __syncthreads();
-            if (active) post_pass_inter(lds, (const float2 *)pl->post, pl->a.n, lt, T,
-                                        (float2 *)sg.out);
+            if (active) post_pass_inter(lds, (const faf_gtw *)pl->post, pl->a.n, lt, T,
+                                        (faf_gf2 *)sg.out);
             break;
         }
         default: {              /* FAFD_SK_CWT: one segment per unit */
@@ -1328,7 +1397,7 @@ __global__ __launch_bounds__(MT) void k_serve(const fafd_splan *plans, const faf
             Lds X{faf_smem};
             Lds W{faf_smem + pl->a.plane};
             const int ah = tid < pl->a.threads, an = tid < pl->b.threads;
-            fft_run<false>(pl->a, GPairIn{sg.in}, X, X, ah ? tid : 0, ah);
+            fft_run<false>(pl->a, SvPairIn{(const faf_gcf *)sg.in}, X, X, ah ? tid : 0, ah);
             __syncthreads();
             if (ah)
                 for (int k = tid; k <= nc / 2; k += pl->a.threads) {
@@ -1339,7 +1408,7 @@ __global__ __launch_bounds__(MT) void k_serve(const fafd_splan *plans, const faf
                         xk = make_float2(z.x + z.y, 0.0f);
                         xm = make_float2(z.x - z.y, 0.0f);
                     } else {
-                        kiss_post(X.ld(k), X.ld(m), ((const float2 *)pl->post)[k], xk, xm);
+                        kiss_post(X.ld(k), X.ld(m), twld((const faf_gtw *)pl->post, k), xk, xm);
                     }
                     X.st(k, xk);
                     X.st(m, xm);
@@ -1347,8 +1416,8 @@ __global__ __launch_bounds__(MT) void k_serve(const fafd_splan *plans, const faf
             __syncthreads();
             const float sc = 1.0f / (float)n;
             for (int r = sg.r0; r < sg.r1; ++r) {
-                OneSidedIn os{X, pl->psi + (long)r * pl->psi_stride, nc};
-                fft_run<true>(pl->b, os, W, GInterOut{(float2 *)sg.out + (long)r * n, sc},
+                SvOneSidedIn os{X, (const faf_gcf *)pl->psi + (long)r * pl->psi_stride, nc};
+                fft_run<true>(pl->b, os, W, SvInterOut{(faf_gf2 *)sg.out + (long)r * n, sc},
                               an ? tid : 0, an);
                 __syncthreads();
             }

@@ -39,29 +39,6 @@ __global__ void kg_fill_u32(unsigned *p, int n, unsigned v) {
  * pass's twiddles too).  Every table is device memory: these give the loads
  * global pointers. */
 FAF_DEV const faf_gtw *gtw(const void *p) { return (const faf_gtw *)p; }
-typedef __attribute__((address_space(1))) const int32_t faf_gi32;
-
-struct GatherLdsG {          /* a[j] = x[rin[j]] from the staged signal */
-    Lds x;
-    const faf_gi32 *rin;
-    static constexpr bool is_lds = true;
-    FAF_DEV float2 ld(int j) const { return x.ld(rin[j]); }
-};
-
-struct MulLdsG {             /* A[j] * B[j] */
-    Lds a;
-    const faf_gtw *b;
-    static constexpr bool is_lds = true;
-    FAF_DEV float2 ld(int j) const { return cmul(a.ld(j), twld(b, j)); }
-};
-
-struct ScatterLdsG {         /* y[rout[j]] = x0 + v  (B carries the 1/m) */
-    Lds y;
-    const faf_gi32 *rout;
-    float2 x0;
-    static constexpr bool is_lds = true;
-    FAF_DEV void st(int j, float2 v) const { y.st(rout[j], cadd(x0, v)); }
-};
 
 /* A unit's signals are addressed from its first signal: one base pointer per
  * plane, the same for the whole workgroup (scalar registers), and a per-lane
@@ -132,13 +109,13 @@ FAF_DEV void rg_rader(const fafd_rplan *pl, const RgIn &src, const RgOut &dst,
         for (int i = lt; i < P; i += T) aa.st(i, src.ld(i));
     __syncthreads();
     const float2 x0 = active ? aa.ld(0) : make_float2(0.0f, 0.0f);
-    GatherLdsG g{aa, (const faf_gi32 *)R.rin};
+    GatherLds g{aa, (const faf_gi32 *)R.rin};
     typename MixFFT<false, P - 1, T, Seq>::Chain fw;
     fw.run_lazy(g, aa, aa, lt, active, gtw(pl->fwd.tw));
     __syncthreads();
     if (active && lt == 0) dst.st(0, cadd(x0, aa.ld(0)));   /* DC = x0 + sum x[1..] */
-    MulLdsG mb{aa, gtw(R.b)};
-    ScatterLdsG sc{aa, (const faf_gi32 *)R.rout, x0};
+    MulLds mb{aa, gtw(R.b)};
+    ScatterLds sc{aa, (const faf_gi32 *)R.rout, x0};
     typename MixFFT<true, P - 1, T, Seq>::Chain bw;
     bw.run_lazy(mb, aa, sc, lt, active, gtw(pl->inv.tw));
     __syncthreads();
