@@ -18,7 +18,9 @@ read off the log.  Bar: relative L2 <= 1e-6 * log2(N) against FP64.
       63's last output rows lie past 2^31 bytes into each plane);
   C4  CWT of 1024 segments of 4096 with 64 scales + lowpass: the first and
       the last segment, against the oracle's FP64 analytic rows built from
-      the reference's own filter table (tests/golden).
+      the reference's own filter table (tests/golden);
+  and a ragged batch whose units must split where their signals would lie
+  2^30 or more elements apart (the kernel's 32-bit unit offsets).
 """
 import ctypes as C
 import sys
@@ -244,3 +246,40 @@ def test_c4_cwt_1024_segments_sampled(golden):
         assert e64 < tol(n)
     finally:
         lib.faf_destroy_transform(t)
+
+
+def test_ragged_units_split_past_32bit_span():
+    """A ragged unit addresses its signals by 32-bit byte offsets from its
+    first signal, so the host ends a unit where the next signal of its length
+    lies 2^30 or more elements further on.  Two 60-point signals separated by
+    131,072 signals of 8192 points (2^30 elements, 4 GiB per plane) would
+    share one unit otherwise; both must come out right, as must the first
+    and last 8192-point signals.  About 17 GB of device memory."""
+    say("case: ragged units past a 32-bit span")
+    lib = G.L()
+    lens = [60] + [8192] * 131072 + [60]
+    arr = (C.c_size_t * len(lens))(*lens)
+    r = lib.faf_ragged_create(arr, len(lens), 0)
+    assert r, lib.faf_get_error()
+    try:
+        tot = lib.faf_ragged_total(r)
+        assert tot == sum(lens) and tot > (1 << 30)
+        gen = torch.Generator(device="cuda").manual_seed(7)
+        xr = torch.empty(tot, device="cuda").uniform_(-1, 1, generator=gen)
+        xi = torch.empty(tot, device="cuda").uniform_(-1, 1, generator=gen)
+        yr, yi = torch.empty_like(xr), torch.empty_like(xi)
+        fa.check(lib.faf_ragged_execute(r, yr.data_ptr(), yi.data_ptr(), xr.data_ptr(),
+                                        xi.data_ptr()), "ragged")
+        torch.cuda.synchronize()
+        off = np.concatenate([[0], np.cumsum(lens)])
+        for i in (0, 1, len(lens) - 2, len(lens) - 1):
+            a, e = int(off[i]), int(off[i + 1])
+            x = (xr[a:e].cpu().numpy() + 1j * xi[a:e].cpu().numpy()).astype(np.complex64)
+            y = yr[a:e].cpu().numpy() + 1j * yi[a:e].cpu().numpy()
+            el = rel_l2(y, O.fft64(x.astype(np.complex128))[0])
+            say(f"  signal {i} (n={lens[i]}, offset {a}): rel l2 {el:.2e}")
+            assert el < tol(lens[i]), (i, el)
+        del xr, xi, yr, yi
+    finally:
+        lib.faf_ragged_destroy(r)
+        torch.cuda.empty_cache()
