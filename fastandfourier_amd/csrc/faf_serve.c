@@ -29,6 +29,7 @@
 #define _POSIX_C_SOURCE 200809L
 #include "faf_internal.h"
 #include <math.h>
+#include <unistd.h>
 #include <pthread.h>
 #include <stdatomic.h>
 #include <stdlib.h>
@@ -43,7 +44,10 @@
 #define SV_CWT_ROWS_PER_UNIT 4
 #define SV_BLOCK_SMALL 512
 #define SV_ALIGN 256
-#define SV_BUILDERS 4            /* plan-building threads */
+/* plan-building threads, at most; fewer on a host with few CPUs.  On a
+ * cold stream of 836 new plans, 4 -> 8 -> 12 builders took the p99 latency
+ * from 435 to 250 to 200 ms. */
+#define SV_BUILDERS 12
 
 enum { ST_QUEUED = 0, ST_ADMITTED, ST_LAUNCHED, ST_DONE, ST_FAILED };
 
@@ -628,6 +632,12 @@ static void build_publish(faf_server *s, sentry *e) {
     pthread_mutex_unlock(&s->in_mu);
 }
 
+/* builder threads to start: SV_BUILDERS, or the online CPUs less two */
+static int builder_cap(void) {
+    const long n = sysconf(_SC_NPROCESSORS_ONLN) - 2;
+    return n < 1 ? 1 : n < SV_BUILDERS ? (int)n : SV_BUILDERS;
+}
+
 static void *builder_main(void *arg) {
     faf_server *s = arg;
     pthread_mutex_lock(&s->plan_mu);
@@ -691,7 +701,7 @@ static sentry *entry_acquire(faf_server *s, int kind, size_t n, int wait) {
             if (s->bq_n < s->bq_cap) {
                 s->bq[(s->bq_head + s->bq_n) % s->bq_cap] = e;
                 s->bq_n++;
-                while (s->nbuilders < SV_BUILDERS &&
+                while (s->nbuilders < builder_cap() &&
                        s->nbuilders < s->bq_n &&
                        pthread_create(&s->builders[s->nbuilders], NULL, builder_main, s) == 0)
                     s->nbuilders++;
