@@ -686,8 +686,10 @@ def run_serving(lib, fa, torch, np, extra, count=100000, rate=200000.0):
     try:
         t0 = time.perf_counter()
         keys = sorted({(int(k), int(n)) for k, n in zip(kinds, ns)})
-        for k, n in keys:
-            fa.check(lib.faf_server_prepare(s, k, n), f"prepare {k} {n}")
+        pk = np.array([k for k, _ in keys], dtype=np.int32)
+        pn = np.array([n for _, n in keys], dtype=np.uint64)
+        fa.check(lib.faf_server_prepare_many(s, pk.ctypes.data, pn.ctypes.data, len(keys)),
+                 "prepare")
         extra["c5_prepare_s"] = round(time.perf_counter() - t0, 2)
         extra["c5_distinct_plans"] = len(keys)
         torch.cuda.synchronize()

@@ -200,6 +200,7 @@ _SIGS = {
     "faf_server_create": (PS, [POINTER(FafServerConfig)]),
     "faf_server_destroy": (None, [PS]),
     "faf_server_prepare": (c_int, [PS, c_int, c_size_t]),
+    "faf_server_prepare_many": (c_int, [PS, c_void_p, c_void_p, c_size_t]),
     "faf_server_set_plain": (c_int, [PS, c_int]),
     "faf_server_set_window": (c_int, [PS, c_double]),
     "faf_server_result_bytes": (c_size_t, [PS, c_int, c_size_t]),

@@ -1379,6 +1379,26 @@ int faf_server_prepare(faf_server *s, faf_req_kind kind, size_t n) {
     return entry_get(s, (int)kind, n) ? 0 : -1;
 }
 
+int faf_server_prepare_many(faf_server *s, const int *kinds, const size_t *ns, size_t count) {
+    if (!s || (count && (!kinds || !ns))) { fafi_err("serve: null argument"); return -1; }
+    /* every new plan goes to the builder threads first, then each is waited
+     * for; the first failure's message is the one reported */
+    int rc = 0;
+    char msg[192] = "";
+    for (size_t i = 0; i < count; ++i)
+        if (!entry_acquire(s, kinds[i], ns[i], 0) && !rc) {
+            rc = -1;
+            snprintf(msg, sizeof msg, "%s", faf_get_error());
+        }
+    for (size_t i = 0; i < count; ++i)
+        if (!entry_acquire(s, kinds[i], ns[i], 1) && !rc) {
+            rc = -1;
+            snprintf(msg, sizeof msg, "%s", faf_get_error());
+        }
+    if (rc) fafi_err("%s", msg);
+    return rc;
+}
+
 size_t faf_server_result_bytes(faf_server *s, faf_req_kind kind, size_t n) {
     if (!s) return 0;
     sentry *e = entry_get(s, (int)kind, n);

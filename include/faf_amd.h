@@ -253,6 +253,10 @@ int faf_server_set_plain(faf_server *s, int on);
 int faf_server_set_window(faf_server *s, double window_us);
 /* build and upload the plan of (kind, n) now rather than at first use */
 int faf_server_prepare(faf_server *s, faf_req_kind kind, size_t n);
+/* the same for count (kinds[i], ns[i]) pairs, built on the server's builder
+ * threads side by side; returns when every plan is ready, 0, or -1 with the
+ * first failure's message (the other plans are still built) */
+int faf_server_prepare_many(faf_server *s, const int *kinds, const size_t *ns, size_t count);
 /* bytes of the result of one (kind, n) request, 0 if not servable */
 size_t faf_server_result_bytes(faf_server *s, faf_req_kind kind, size_t n);
 /* Returns a ticket >= 0, or -1 (faf_get_error) for a request that is wrong

@@ -393,3 +393,33 @@ def test_cwt_long_segments_match_fp64(n):
             assert rel_l2(wr[sgm], ref.real) <= tol(n)
     finally:
         lib.faf_destroy_transform(t)
+
+
+def test_serve_prepare_many():
+    """faf_server_prepare_many builds a list of plans side by side and returns
+    when all are ready; a plan that cannot be built makes it return -1 with
+    that plan's message while the others are still built and serve at once."""
+    say("case: serve prepare many")
+    lib = G.L()
+    rng = np.random.default_rng(41)
+    s = server(hbm_budget=1 << 30)
+    try:
+        cases = [("rfft", 4096), ("c2c", 1000), ("stft", 4000), ("cwt", 8192), ("c2c", 1009)]
+        kid = {"rfft": 0, "c2c": 1, "stft": 2, "cwt": 3}
+        ks = np.array([kid[k] for k, _ in cases] + [1], dtype=np.int32)
+        ns = np.array([n for _, n in cases] + [65521], dtype=np.uint64)
+        assert lib.faf_server_prepare_many(s, ks.ctypes.data, ns.ctypes.data, len(ks)) == -1
+        assert lib.faf_get_error()
+        fa.check(lib.faf_server_prepare_many(s, ks.ctypes.data, ns.ctypes.data, len(ks) - 1),
+                 "prepare many")
+        for kind, n in cases:
+            x, dx = make_input(rng, kind, n)
+            t = lib.faf_server_submit(s, kid[kind], n, dx.data_ptr(), 0)
+            assert t >= 0, lib.faf_get_error()
+            err = rel_l2(fetch(s, t), truth(kind, n, x))
+            say(f"  {kind} {n}: rel l2 {err:.2e}")
+            assert err < bar(kind, n), (kind, n, err)
+            fa.check(lib.faf_server_release(s, t), "release")
+        assert lib.faf_server_prepare_many(s, None, None, 0) == 0
+    finally:
+        lib.faf_server_destroy(s)
