@@ -643,7 +643,10 @@ static void *builder_main(void *arg) {
     pthread_mutex_lock(&s->plan_mu);
     for (;;) {
         while (!s->bq_n && !s->build_stop) pthread_cond_wait(&s->build_cv, &s->plan_mu);
-        if (!s->bq_n) break;                 /* stopping and nothing left */
+        /* stopping: builds still queued are dropped (the server is being
+         * destroyed and its worker has already failed the requests parked
+         * on them); unbuilt entries are freed with the rest */
+        if (s->build_stop) break;
         sentry *e = s->bq[s->bq_head];
         s->bq_head = (s->bq_head + 1) % s->bq_cap;
         s->bq_n--;
@@ -1259,7 +1262,7 @@ void faf_server_destroy(faf_server *s) {
         pthread_mutex_unlock(&s->in_mu);
         pthread_join(s->th, NULL);
     }
-    pthread_mutex_lock(&s->plan_mu);          /* builders finish what they hold */
+    pthread_mutex_lock(&s->plan_mu);          /* builders finish the build they hold */
     s->build_stop = 1;
     pthread_cond_broadcast(&s->build_cv);
     pthread_mutex_unlock(&s->plan_mu);

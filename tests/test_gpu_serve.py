@@ -423,3 +423,41 @@ def test_serve_prepare_many():
         assert lib.faf_server_prepare_many(s, None, None, 0) == 0
     finally:
         lib.faf_server_destroy(s)
+
+
+def _smooth5(n):
+    for p in (2, 3, 5):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+def test_serve_destroy_while_plans_build():
+    """Destroying a server while new plans are still queued for its builder
+    threads fails the requests waiting on them, drops the queued builds and
+    returns promptly; a new server then serves normally."""
+    import time
+    say("case: serve destroy during builds")
+    lib = G.L()
+    rng = np.random.default_rng(43)
+    s = server(hbm_budget=1 << 30)
+    x, dx = make_input(rng, "rfft", 65536)
+    smooth = [n for n in range(20000, 65537, 2) if _smooth5(n)]
+    tickets = []
+    for n in smooth[-40:]:                           # 40 new CWT banks (slow builds)
+        t = lib.faf_server_submit(s, 3, n, dx.data_ptr(), 0)
+        assert t >= 0, lib.faf_get_error()
+        tickets.append(t)
+    t0 = time.perf_counter()
+    lib.faf_server_destroy(s)
+    took = time.perf_counter() - t0
+    say(f"  destroy took {took:.2f} s with {len(tickets)} requests submitted")
+    assert took < 20.0
+    s = server(hbm_budget=1 << 28)
+    try:
+        t = lib.faf_server_submit(s, 0, 4096, dx.data_ptr(), 0)
+        assert t >= 0, lib.faf_get_error()
+        err = rel_l2(fetch(s, t), truth("rfft", 4096, x[:4096]))
+        assert err < bar("rfft", 4096), err
+    finally:
+        lib.faf_server_destroy(s)
