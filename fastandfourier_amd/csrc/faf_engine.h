@@ -600,7 +600,8 @@ __global__ __launch_bounds__(MT) void k_c2r(const fafd_fft *Pd, int spb, const f
 /* the op list that is exactly one MUL by the bound spectrum (the notch and
  * filter pipelines): apply_ops without the per-bin interpretation */
 FAF_DEV float2 op_mul(const fafd_ops &ops, int k, float2 v) {
-    const long m = ops.h_bins;
+    /* a bin index is an int: the bound compares in 32 bits (per bin, twice) */
+    const int m = ops.h_bins < 0x7fffffffL ? (int)ops.h_bins : 0x7fffffff;
     if (k < m) {
         if (k == 0 || k == m - 1) v = make_float2(v.x * ops.h_re[k], 0.0f);
         else v = cmul(v, make_float2(ops.h_re[k], ops.h_im[k]));
