@@ -1,6 +1,6 @@
 """Benchmark of the transform execution path on one MI355X.
 
-Headline (the workload BASELINE names): batched R2C rfft, N=4096, FP32 real
+Headline: batched R2C rfft, N=4096, FP32 real
 input N(0,1), batch 2^28/N = 65536 signals (1 GiB in), Hermitian-packed
 N/2+1 bins out.  A step is one batched rfft launch over the whole batch with
 inputs resident in HBM.  value = algorithmic bytes per step / step time,
@@ -114,9 +114,9 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU: each box has one MI355X; "
-                                   "the batch shards across GPUs with no exchange "
-                                   "(see DESIGN.md), run with --gpus 1"}))
+        print(json.dumps({"error": "this benchmark measures one GPU; the batch would shard "
+                                   "across GPUs as independent slices of signals with no "
+                                   "exchange; run with --gpus 1"}))
         sys.exit(2)
     import numpy as np
     import torch
