@@ -272,7 +272,7 @@ def main():
     extra["plain_per_call_rfft_4096_gbs"] = round(pc * (4 * n + 8 * nb) / (ms_plain * 1e-3) / 1e9, 2)
     extra["batched_over_plain"] = round(gbs / extra["plain_per_call_rfft_4096_gbs"], 1)
 
-    # copy ceilings on this box (read + write of 1 GiB): torch's copy (the
+    # copy ceilings on this board (read + write of 1 GiB): torch's copy (the
     # runtime's blit kernel) and the library's 16-byte streaming copy, at a
     # few grid sizes (best kept)
     z = torch.empty_like(x)
