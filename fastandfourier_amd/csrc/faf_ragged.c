@@ -500,13 +500,15 @@ int faf_ragged_execute(faf_ragged *r, float *out_re, float *out_im, const float 
 
 #define RG_MAX_CANDS 48
 
-/* median launch time (ms) of r over 5 timed launches after 2 warm ones */
-static float time_ragged(faf_ragged *r, float *buf, void *stream) {
+/* median launch time (ms) of r over `reps` (odd, at most 9) timed launches
+ * after 2 warm ones */
+static float time_ragged(faf_ragged *r, float *buf, void *stream, int reps) {
     float *ir = buf, *ii = buf + r->total, *orr = buf + 2 * r->total, *oi = buf + 3 * r->total;
     for (int w = 0; w < 2; ++w)
         if (launch(r, orr, oi, ir, ii, stream)) return -1.0f;
-    float t[5];
-    for (int k = 0; k < 5; ++k) {
+    float t[9];
+    if (reps > 9) reps = 9;
+    for (int k = 0; k < reps; ++k) {
         void *e0 = fafk_event_create(), *e1 = fafk_event_create();
         fafk_event_record(e0, stream);
         int rc = launch(r, orr, oi, ir, ii, stream);
@@ -516,9 +518,9 @@ static float time_ragged(faf_ragged *r, float *buf, void *stream) {
         fafk_event_destroy(e1);
         if (t[k] < 0.0f) return -1.0f;
     }
-    for (int i = 1; i < 5; ++i)         /* insertion sort, then the median */
+    for (int i = 1; i < reps; ++i)      /* insertion sort, then the median */
         for (int j = i; j > 0 && t[j] < t[j - 1]; --j) { float s = t[j]; t[j] = t[j - 1]; t[j - 1] = s; }
-    return t[2];
+    return t[reps / 2];
 }
 
 /* candidate choices of one length */
@@ -665,7 +667,7 @@ int faf_ragged_autotune(faf_ragged *r) {
             rg_overrides ov = {&uniq[k], &c, 1};
             faf_ragged *trial = ragged_new(one, m, r->flags, &ov, 0);
             if (!trial) continue;            /* this length cannot take that choice */
-            float ms = time_ragged(trial, buf, stream);
+            float ms = time_ragged(trial, buf, stream, 5);
             const int lean = !trial->generic;
             faf_ragged_destroy(trial);
             if (ms <= 0.0f) continue;
@@ -705,7 +707,9 @@ int faf_ragged_autotune(faf_ragged *r) {
                 if (a && !bw) auto_block = a->block;
                 for (int ps = 0; a && ps < 2; ++ps) {  /* one workgroup per unit, persistent */
                     a->persist = ps;
-                    const float ta = time_ragged(a, buf, stream);
+                    /* the whole-batch candidates differ by a few percent:
+                     * a median of 9 picks among them more reliably */
+                    const float ta = time_ragged(a, buf, stream, 9);
                     if (ta > 0.0f && ta < twin) {
                         twin = ta;
                         win = cand[c];

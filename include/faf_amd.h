@@ -98,9 +98,10 @@ void faf_ragged_destroy(faf_ragged *r);
  * on the device and on a batch of that length alone, every way the length
  * can run (compile-time passes, or runtime radix passes in each candidate
  * radix order) with several signals-per-unit counts, median of 5 launches
- * each; then it times the whole batch with the per-length winners against
- * the best compile-time choices (the launch without runtime paths uses a
- * leaner kernel) and keeps the faster.  The winners are recorded for the
+ * each; then it times the whole batch (median of 9) with the per-length
+ * winners, the best compile-time choices (the launch without runtime paths
+ * uses a leaner kernel) and the defaults, in both launch shapes and both
+ * workgroup sizes, and keeps the fastest.  The winners are recorded for the
  * process, so later faf_ragged_create calls use them for those lengths;
  * faf_ragged_create with FAF_FLAG_MEASURE in flags runs this at create.
  * Rebuilds r in place.  Returns the number of lengths timed (0 when every
